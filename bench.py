@@ -1,0 +1,150 @@
+"""XPFC time-stepping throughput on one MI355X.
+
+Workload: 3-D fcc two-peak XPFC, 512^3 periodic grid, dx = 0.125, dt = 0.5,
+peaks k1 = 2π, k2 = 4π/√3 (α = 1, σ = 0.1, max envelope), n0 = 0.05 + 0.01 N(0,1)
+(seed 2), FP32 fields and FFTs with FP64 denominator / k = 0 mode / reductions.
+A step is one full semi-implicit update of the field resident in HBM (the
+reference's loop body, main.cu:306-435); FE/RMSE are sampled every 100 steps.
+
+Prints one JSON line.  `value` = steps/s of the FP32-mixed path; the FP64
+plain path at the same size is timed beside it (`fp64_steps_per_s`).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def fcc_params(n, prec, t_sample=100):
+    import xpfc_cuda_amd as X
+    k1 = 2.0 * math.pi
+    k2 = k1 * 2.0 / math.sqrt(3.0)
+    L = n * 0.125
+    return X.Params(n=(n, n, n), L=(L, L, L), dt=0.5, Mo=1.0, eps=0.0, a=1.0, b=1.0 / 3.0,
+                    sigma=0.1, psi0=0.05, peaks=((k1, 1.0, 1.0), (k2, 1.0, 1.0)),
+                    dcf_mode=X.XPFC_DCF_MAX, precision=prec, t_sample=t_sample)
+
+
+def spectrum_bytes(n, prec):
+    # packed half spectrum: n*n*(n/2) main + n*n Nyquist plane, complex
+    csize = 8 if prec == 1 else 16
+    return n * n * (n // 2 + 1) * csize
+
+
+def gpu_run(n, prec, steps, warmup, seed=2):
+    import xpfc_cuda_amd as X
+    p = fcc_params(n, prec)
+    with X.Context(p) as ctx:
+        ctx.initialize_noise(0.01, seed)
+        if warmup:
+            ctx.step(warmup)
+        ctx.sync()
+        t0 = time.perf_counter()
+        ms = ctx.time_steps(steps)
+        ctx.sync()
+        wall = time.perf_counter() - t0
+        passes = ctx.time_passes(max(5, min(steps, 50)))
+        samples = ctx.samples()
+    return ms, wall, passes, samples
+
+
+def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
+    """The FP64 oracle (numpy/scipy pocketfft, all host threads) on a bounded
+    number of steps of the same 512^3 workload; steps/s."""
+    import numpy as np
+    from oracle import xpfc_oracle as O
+    workers = os.cpu_count() or 1
+    k1 = 2.0 * math.pi
+    k2 = k1 * 2.0 / math.sqrt(3.0)
+    L = n * 0.125
+    op = O.Params(n=(n, n, n), L=(L, L, L), sigma=0.1, psi0=0.05,
+                  peaks=[O.Peak(k1, 1.0, 1.0), O.Peak(k2, 1.0, 1.0)], dcf_mode="max",
+                  t_sample=100)
+    psi = op.psi0 + 0.01 * np.random.default_rng(2).standard_normal(op.n)
+    # one step to warm caches/plans, then time up to nsteps_cap steps within the budget
+    O.run(psi, op, 1, sample=False, workers=workers)
+    done = 0
+    t0 = time.perf_counter()
+    while done < nsteps_cap and time.perf_counter() - t0 < budget_s:
+        psi, _ = O.run(psi, op, 1, sample=False, workers=workers)
+        done += 1
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "steps/s", "cores": workers, "kind": "port",
+            "sample": "%d step(s) of the %d^3 FP64 oracle (scipy.fft, %d workers) on %s"
+                      % (done, n, workers, platform.processor() or platform.machine())}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the workload would shard as x-slabs with two all-to-all transposes per step)",
+              file=sys.stderr)
+        sys.exit(2)
+    n = args.n
+    ms, wall, passes, samples = gpu_run(n, 1, args.steps, args.warmup)
+    ms_per_step = ms / args.steps
+    value = 1000.0 / ms_per_step
+    S = spectrum_bytes(n, 1)
+    # dominant kernel: the axis-0 fused pass, 4 S algorithmic bytes per launch
+    x_ms = passes[0]
+    achieved = 4.0 * S / (x_ms * 1e-3) / 1e9
+    step_gbs = 10.0 * S / (passes[4] * 1e-3) / 1e9
+    out = {
+        "metric": "XPFC steps/s at 512^3 (FP32-mixed)",
+        "value": round(value, 3),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 denominator, k=0 mode, reductions)",
+        "data": "synthetic: n0=0.05+0.01*N(0,1), seed 2",
+        "config": {"workload": "3-D fcc two-peak XPFC %d^3, dx=0.125, dt=0.5, sigma=0.1, "
+                               "FP32-mixed, FE/RMSE every 100 steps" % n,
+                   "grid": [n, n, n], "parallelism": "single GPU"},
+        "cell_updates_per_s": value * n ** 3,
+        "step_hbm_gbs": round(step_gbs, 1),
+        "pass_ms": {"x_fused": passes[0], "y_inv": passes[1], "z_real": passes[2],
+                    "y_fwd": passes[3], "step": passes[4]},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                     "traffic": None,
+                     "kernel": "k_xstep (axis-0 FFT + update + inverse FFT)",
+                     "algorithmic_bytes": 4 * S},
+        "samples": [s.line().strip() for s in samples[:4]],
+    }
+    if not args.no_fp64:
+        fsteps = max(10, args.steps // 4)
+        ms64, _, passes64, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
+        v64 = 1000.0 / (ms64 / fsteps)
+        out["fp64_steps_per_s"] = round(v64, 3)
+        out["fp32_over_fp64"] = round(value / v64, 3)
+        out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(n)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,193 @@
+/*
+ * xpfc.h -- C ABI of the MI355X XPFC time-stepping library (libxpfc.so).
+ *
+ * The reference (LC2500/XPFC_CUDA) has no library: its operator API is the set
+ * of free functions declared in globals.cuh:98-145 and driven by main.cu over
+ * process globals (globals.cuh:36-96).  Each entry point below names the
+ * reference function it replaces.  Differences that are deliberate:
+ *   - state lives in an opaque context instead of globals;
+ *   - errors are returned as status codes (never exit()), see xpfc_strerror;
+ *   - results are returned to the caller instead of printed from the device
+ *     (sum_reduction.cu:64,77);
+ *   - fields are real arrays indexed [i][j] (2-D) or [i][j][l] (3-D), last
+ *     index contiguous, exactly like psi_host[i*My + j] (initialize.cu:105);
+ *   - spectra at the operator level use the real-transform half layout
+ *     [n0][n1][n2/2+1] (the reference keeps full Z2Z arrays whose imaginary
+ *     part of the inverse is discarded by copy_matrix.cu:44-45).
+ * All pointers are plain host or device pointers; sizes are element counts.
+ * A context is not re-entrant; distinct contexts may be used from distinct
+ * host threads.
+ */
+#ifndef XPFC_H_
+#define XPFC_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define XPFC_OK 0
+#define XPFC_ERR_ARG 1         /* invalid argument or unsupported size      */
+#define XPFC_ERR_IO 2          /* file could not be opened/read/written     */
+#define XPFC_ERR_PARSE 3       /* malformed pfc.input or VTK                */
+#define XPFC_ERR_DEVICE 4      /* HIP runtime error                         */
+#define XPFC_ERR_NOMEM 5       /* device or host allocation failed          */
+#define XPFC_ERR_STATE 6       /* call out of order (e.g. no field set)     */
+#define XPFC_ERR_NONFINITE 7   /* a reduction produced NaN/Inf              */
+
+#define XPFC_FP64 0            /* every field and transform in FP64          */
+#define XPFC_FP32_MIXED 1      /* FP32 fields/FFTs; FP64 denominator, k=0, sums */
+
+#define XPFC_DCF_SUM 0         /* initialize.cu:215-217 (reference active)   */
+#define XPFC_DCF_MAX 1         /* initialize.cu:209-212 (envelope)           */
+
+#define XPFC_MAX_PEAKS 4
+
+/* Parameters.  Field names follow read_input.cu:19-115 and globals.cuh:40-52. */
+typedef struct xpfc_params {
+  int dim;                   /* 2 or 3 (the reference is 2-D, globals.cuh:26) */
+  int n[3];                  /* Mx, My, Mz (Mz = 1 in 2-D); powers of two, 4..1024 */
+  double L[3];               /* Lx, Ly, Lz box lengths (ints in the reference) */
+  double dt;                 /* time step                                       */
+  double Mo;                 /* mobility                                        */
+  double eps;                /* epsilon of the denominator (euler_step.cu:54)   */
+  double a, b;               /* a = 1/a_in, b = 1/b_in (read_input.cu:28)       */
+  double sigma;              /* s: DCF temperature parameter                    */
+  double psi0;               /* mean density psi_0                              */
+  int npeaks;                /* DCF peaks (initialize.cu:10-17 hard-codes 2)    */
+  double peak_k[XPFC_MAX_PEAKS];
+  double peak_alpha[XPFC_MAX_PEAKS];
+  double peak_rho_beta[XPFC_MAX_PEAKS]; /* rho_i * B_i of exp(-(s k_i)^2/(2 rho_i B_i)) */
+  int dcf_mode;              /* XPFC_DCF_SUM or XPFC_DCF_MAX                    */
+  int precision;             /* XPFC_FP64 or XPFC_FP32_MIXED                    */
+  int t_sample;              /* sampling period in steps (0 = never)           */
+  int int_Mo_residual;       /* 1: residual uses (int)Mo (sse_calculation.cu:120) */
+  /* pfc.input fields kept for the CLI and initial conditions */
+  int no_val;
+  int t_final;               /* t (int): the reference runs t/dt + 1 steps      */
+  int iconfig;               /* 0 tri, 1 square, 2 liquid, 10 VTK restart       */
+  double A_10, A_11, A_tri;
+  int iseed, ishape;
+  double h, k, roa;
+  int device;                /* HIP device ordinal                              */
+} xpfc_params;
+
+/* One sampled diagnostic record (what sum_reduction.cu:64,77 prints). */
+typedef struct xpfc_sample {
+  long long step;            /* n of "iteration: n"                             */
+  int has_fe;                /* FE of the field after step-1 (main.cu:412-426)  */
+  int has_sse;               /* RMSE of the update n -> n+1 (main.cu:326-328)   */
+  double F_ref;              /* ff_factor * sum f   (FE_calc.cu:64, initialize.cu:63) */
+  double f_mean;             /* sum f / N                                        */
+  double mean;               /* real-space sum psi / N (Avg_calc.cu:3-18)       */
+  double rmse;               /* sqrt(sum R^2 / N)   (sum_reduction.cu:77)       */
+} xpfc_sample;
+
+typedef struct xpfc_ctx xpfc_ctx;
+
+/* ---- parameters ---------------------------------------------------------- */
+/* Defaults: the reference's hard-coded DCF (initialize.cu:10-17, sum mode),
+ * a = 1, b = 1/3, Mo = 1, eps = 0, dt = 0.5, FP64. */
+int xpfc_params_default(xpfc_params* out);
+/* read_input (read_input.cu:8-121): parse a pfc.input file, same positional
+ * format; a = 1/a_in, b = 1/b_in.  Returns XPFC_ERR_IO / XPFC_ERR_PARSE. */
+int xpfc_read_input(const char* path, xpfc_params* out);
+/* N_t_steps = t/dt + 1 truncated as the loop bound uses it (initialize.cu:60). */
+long long xpfc_reference_steps(const xpfc_params* p);
+
+/* ---- context ------------------------------------------------------------- */
+/* Allocates the device fields, twiddle tables and stream (main.cu:106-207). */
+int xpfc_create(const xpfc_params* p, xpfc_ctx** out);
+void xpfc_destroy(xpfc_ctx* ctx);
+int xpfc_get_params(const xpfc_ctx* ctx, xpfc_params* out);
+
+/* ---- initial conditions (initialize.cu:95-202, randn.cu:4-12) ------------ */
+/* Reference initial condition from p->iconfig/iseed/ishape (2-D), with the
+ * average-density fix of Avg_calc.cu:21-43; uploads the field. */
+int xpfc_initialize(xpfc_ctx* ctx);
+/* psi = psi0 + amplitude * randn(seed, cell), mean re-fixed to psi0. */
+int xpfc_initialize_noise(xpfc_ctx* ctx, double amplitude, unsigned long long seed);
+/* randn: count N(0,1) draws for cells first..first+count-1 (Philox4x32-10 +
+ * Box-Muller, FP64).  Replaces the unseeded randn() of randn.cu:4-12. */
+int xpfc_randn(unsigned long long seed, unsigned long long first, long long count, double* out);
+/* Philox4x32-10 block function (exposed for known-answer tests). */
+void xpfc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+
+/* ---- state I/O (main.cu:226 H2D, copy_to_cpu.cu:3-18 D2H) ---------------- */
+int xpfc_set_field(xpfc_ctx* ctx, const double* host_real);   /* N values */
+int xpfc_get_field(xpfc_ctx* ctx, double* host_real);         /* N values */
+long long xpfc_num_cells(const xpfc_ctx* ctx);
+
+/* ---- the time-stepping hot path (main.cu:306-435) ------------------------ */
+/* Advance nsteps steps.  Every t_sample steps the FE / RMSE records of the
+ * reference schedule (SSE when n % t_sample == 0, FE when (n+1) % t_sample
+ * == 0) are appended to the sample log. */
+int xpfc_step(xpfc_ctx* ctx, int nsteps);
+long long xpfc_current_step(const xpfc_ctx* ctx);
+/* Sample log: number of records, copy up to max of them, clear. */
+int xpfc_num_samples(const xpfc_ctx* ctx);
+int xpfc_get_samples(const xpfc_ctx* ctx, xpfc_sample* out, int max);
+void xpfc_clear_samples(xpfc_ctx* ctx);
+/* Time nsteps steps with hipEvents on the context stream (helpers_header.cuh:12-48). */
+int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
+/* Per-pass timing: run nsteps unsampled steps with a hipEvent between every
+ * launch; ms[0..3] = average ms of the axis-0 fused pass, the axis-1 inverse
+ * pass, the real-space pass and the axis-1 forward pass (0 in 2-D), ms[4] =
+ * average ms per whole step. */
+int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
+/* Block until all work of the context has finished. */
+int xpfc_sync(xpfc_ctx* ctx);
+
+/* ---- diagnostics ----------------------------------------------------------- */
+/* FE_calc_wrapper (FE_calc.cu:3-51) + convolution_wrapper (convolve_fields.cu:3-27)
+ * for the current field: F_ref = ff_factor * sum f and f_mean = sum f / N. */
+int xpfc_fe_calc(xpfc_ctx* ctx, double* F_ref, double* f_mean);
+/* mean / Avg_calc (Avg_calc.cu:3-18): the FP64 mean density. */
+int xpfc_avg_calc(xpfc_ctx* ctx, double* mean);
+/* SSE_calc_wrapper (sse_calculation.cu:3-116): RMSE of the last sampled update. */
+int xpfc_sse_calc(xpfc_ctx* ctx, double* rmse);
+
+/* ---- operator level (device pointers; element type = context precision) ---- */
+int xpfc_malloc(xpfc_ctx* ctx, void** dptr, size_t bytes);
+int xpfc_free(xpfc_ctx* ctx, void* dptr);
+int xpfc_memcpy_h2d(xpfc_ctx* ctx, void* dst, const void* src, size_t bytes);
+int xpfc_memcpy_d2h(xpfc_ctx* ctx, void* dst, const void* src, size_t bytes);
+size_t xpfc_real_bytes(const xpfc_ctx* ctx);      /* N reals                      */
+size_t xpfc_spectrum_bytes(const xpfc_ctx* ctx);  /* n0*n1*(n2/2+1) complex       */
+/* power_function_wrapper (power_functions.cu:4-53): dst = src^p on real fields
+ * (exact products in the working precision; the reference's FP32 powf is an
+ * oracle switch). */
+int xpfc_power_functions(xpfc_ctx* ctx, int p, const void* src, void* dst);
+/* convolution_wrapper (convolve_fields.cu:3-51): out = c2(k) * psi_k. */
+int xpfc_convolve_fields(xpfc_ctx* ctx, const void* psi_k, void* out);
+/* euler_step_wrapper (euler_step.cu:3-63): out = semi-implicit update. */
+int xpfc_euler_step(xpfc_ctx* ctx, const void* psi_k, const void* p2_k, const void* p3_k, void* out);
+/* residual_calc (sse_calculation.cu:120-147): the k-space residual. */
+int xpfc_residual_calc(xpfc_ctx* ctx, const void* psi_kp1, const void* psi_k, const void* p2_k,
+                       const void* p3_k, void* out);
+/* scaling_step_wrapper (scaling_step.cu:4-53): buf /= N (spectral=1: a
+ * half spectrum; 0: a real field). */
+int xpfc_scaling_step(xpfc_ctx* ctx, void* buf, int spectral);
+/* cufftXtExecDescriptor FORWARD / INVERSE on real data (main.cu:309,343): the
+ * unnormalised real-to-half-spectrum transform and its unnormalised inverse. */
+int xpfc_fft_r2c(xpfc_ctx* ctx, const void* real_in, void* spec_out);
+int xpfc_fft_c2r(xpfc_ctx* ctx, const void* spec_in, void* real_out);
+
+/* ---- output (write_vtk.cu:4-35) ------------------------------------------- */
+int xpfc_write_vtk(xpfc_ctx* ctx, const char* path);
+/* Restart input (read_input.cu:161-241): read a 2-D legacy VTK field into
+ * host memory (values in file order), dims returned. */
+int xpfc_read_vtk(const char* path, int* nx, int* ny, double* spacing, double* host_real,
+                  long long capacity);
+
+/* ---- errors ----------------------------------------------------------------- */
+const char* xpfc_strerror(int status);
+const char* xpfc_last_error(void);   /* detail of the last failure on this thread */
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* XPFC_H_ */

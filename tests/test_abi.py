@@ -1,0 +1,128 @@
+"""The C ABI library: loads without a GPU, exports every declared symbol, and
+its host-only entry points (params, read_input, Philox/randn) behave."""
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import xpfc_cuda_amd as X
+from oracle import xpfc_oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "xpfc.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(xpfc_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = X.load_library()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_params_default_matches_reference_constants():
+    p = X.Params()
+    c = p.to_c()
+    d = X.api._Params()
+    import ctypes
+    assert X.lib().xpfc_params_default(ctypes.byref(d)) == 0
+    # initialize.cu:10-17
+    assert d.npeaks == 2
+    assert d.peak_k[0] == 2.0 * math.pi and d.peak_k[1] == 2.0 * math.pi * math.sqrt(2.0)
+    assert d.peak_rho_beta[0] == 4.0 and d.peak_rho_beta[1] == 4.0 / math.sqrt(2.0)
+    assert d.dcf_mode == X.XPFC_DCF_SUM
+    assert c.dim == 2 and c.n[2] == 1
+
+
+# Random123 known-answer vectors for Philox4x32-10 (Salmon et al., SC'11)
+KAT = [
+    ((0, 0, 0, 0), (0, 0), (0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8)),
+    ((0xFFFFFFFF,) * 4, (0xFFFFFFFF,) * 2, (0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD)),
+    ((0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344), (0xA4093822, 0x299F31D0),
+     (0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1)),
+]
+
+
+@pytest.mark.parametrize("ctr,key,expect", KAT)
+def test_philox_kat_library(ctr, key, expect):
+    assert tuple(X.philox4x32_10(ctr, key)) == expect
+
+
+def test_randn_library_matches_oracle():
+    seed = 12345678901
+    got = X.randn(seed, 4096, first=0)
+    ref = O.philox_normals(4096, seed)
+    assert np.max(np.abs(got - ref)) <= 4e-15
+    # offset draws are the same stream
+    assert np.array_equal(X.randn(seed, 10, first=100), got[100:110])
+
+
+PFC_INPUT = """1          no_val
+0.05 0.1   psi_0 s
+1.0 3.0 0.0 1.0  a b eps Mo
+
+32 32      Lx Ly
+256 256    Mx My
+
+500        t
+0.5        dt
+100        t_sample
+
+1          iconfig
+0.1 0.05   A_10 A_11
+0.2        A_tri
+
+1          iseed
+1          ishape
+16.0 16.0 9.0    h k roa
+"""
+
+
+def test_read_input_reference_format(tmp_path):
+    f = tmp_path / "pfc.input"
+    f.write_text(PFC_INPUT)
+    p = X.read_input(str(f))
+    assert p.n == (256, 256) and p.L == (32.0, 32.0)
+    assert p.a == 1.0 and p.b == pytest.approx(1.0 / 3.0)  # read_input.cu:28
+    assert p.psi0 == 0.05 and p.sigma == 0.1 and p.dt == 0.5 and p.t_sample == 100
+    assert p.iconfig == 1 and p.A_10 == 0.1 and p.A_11 == 0.05 and p.A_tri == 0.2
+    assert (p.iseed, p.ishape, p.h, p.k, p.roa) == (1, 1, 16.0, 16.0, 9.0)
+    assert X.reference_steps(p) == 1001  # t/dt + 1
+
+
+def test_read_input_triangular_box_truncates(tmp_path):
+    f = tmp_path / "pfc.input"
+    f.write_text(PFC_INPUT.replace("1          iconfig", "0          iconfig"))
+    p = X.read_input(str(f))
+    assert p.L == (float(int(32 * 2 / math.sqrt(3))),) * 2  # initialize.cu:25
+
+
+def test_read_input_errors(tmp_path):
+    with pytest.raises(X.XPFCError) as e:
+        X.read_input(str(tmp_path / "missing.input"))
+    assert e.value.status == 2
+    f = tmp_path / "bad.input"
+    f.write_text(PFC_INPUT.replace("1          iconfig", "7          iconfig"))
+    with pytest.raises(X.XPFCError) as e:
+        X.read_input(str(f))
+    assert e.value.status == 3
+    f.write_text("1\n0.05\n")
+    with pytest.raises(X.XPFCError) as e:
+        X.read_input(str(f))
+    assert e.value.status == 3
+
+
+def test_read_input_extensions(tmp_path):
+    f = tmp_path / "pfc.input"
+    f.write_text(PFC_INPUT + "dim 3\nLz 16\nMz 128\nprecision fp32\ndcf max\npeaks 1 6.283185307179586 1.0 1.0\n")
+    p = X.read_input(str(f))
+    assert p.n == (256, 256, 128) and p.L[2] == 16.0
+    assert p.precision == X.XPFC_FP32_MIXED and p.dcf_mode == X.XPFC_DCF_MAX
+    assert len(p.peaks) == 1 and p.peaks[0][2] == 1.0

@@ -1,0 +1,166 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances: FP64 path -- relative to the field scale 1e-11 (transforms) and
+1e-10 (many steps); FP32-mixed path -- 2e-6 per transform, max|Δn| <= 1e-5 over
+the step counts here.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import xpfc_oracle as O
+from xpfc_cuda_amd import Context, XPFC_FP32_MIXED, XPFC_FP64
+
+from cases import growth_case, make
+
+pytestmark = pytest.mark.gpu
+
+PRECS = [XPFC_FP64, XPFC_FP32_MIXED]
+TOL_FFT = {XPFC_FP64: 1e-12, XPFC_FP32_MIXED: 2e-6}
+
+SHAPES = [(16, 32), (64, 64), (256, 256), (1024, 64), (8, 16, 32), (32, 32, 32), (64, 16, 128),
+          (128, 128, 128)]
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+@pytest.mark.parametrize("prec", PRECS)
+def test_fft_r2c_c2r_vs_numpy(shape, prec):
+    _flush("fft %s prec=%d" % (shape, prec))
+    _, lp = make(shape, prec=prec)
+    rng = np.random.default_rng(7)
+    x = rng.standard_normal(shape)
+    with Context(lp) as ctx:
+        X = ctx.fft_r2c(x)
+        ref = np.fft.rfftn(x)
+        scale = np.max(np.abs(ref))
+        assert np.max(np.abs(X - ref)) <= TOL_FFT[prec] * scale * 4
+        # unnormalised inverse: c2r(r2c(x)) = N x
+        back = ctx.fft_c2r(ref)
+        N = x.size
+        assert np.max(np.abs(back / N - x)) <= TOL_FFT[prec] * np.max(np.abs(x)) * 8
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_set_get_field_round_trip(prec):
+    _, lp = make((64, 32, 16), prec=prec)
+    rng = np.random.default_rng(3)
+    x = 0.05 + 0.01 * rng.standard_normal(lp.n)
+    with Context(lp) as ctx:
+        ctx.set_field(x)
+        y = ctx.get_field()
+        assert np.max(np.abs(y - x)) <= (1e-15 if prec == XPFC_FP64 else 1e-8)
+        assert ctx.avg_calc() == pytest.approx(float(np.sum(x) / x.size), rel=0, abs=1e-17)
+
+
+def _run_both(op, lp, nsteps, amp=0.01, seed=0):
+    psi0 = O.noise_ic(op, amp, seed)
+    ref, rs = O.run(psi0, op, nsteps, sample=True)
+    with Context(lp) as ctx:
+        ctx.set_field(psi0)
+        ctx.step(nsteps)
+        got = ctx.get_field()
+        gs = ctx.samples()
+    return psi0, ref, rs, got, gs
+
+
+@pytest.mark.parametrize("shape,nsteps", [((32, 32), 60), ((64, 64), 80), ((16, 16, 16), 40),
+                                          ((32, 16, 64), 30)])
+@pytest.mark.parametrize("prec", PRECS)
+def test_steps_vs_oracle_growth(shape, nsteps, prec):
+    _flush("steps %s n=%d prec=%d" % (shape, nsteps, prec))
+    op, lp = growth_case(shape, prec=prec, t_sample=10)
+    psi0, ref, rs, got, gs = _run_both(op, lp, nsteps)
+    moved = np.max(np.abs(ref - psi0))
+    assert moved > 1e-3  # the field actually evolved
+    err = np.max(np.abs(got - ref))
+    tol = 1e-10 if prec == XPFC_FP64 else 1e-5
+    assert err <= tol, (err, moved)
+    # samples: same schedule, FE and RMSE values
+    rfe = {s.step: s for s in rs if s.kind == "FE"}
+    rsse = {s.step: s.value for s in rs if s.kind == "SSE"}
+    gfe = {s.step: s for s in gs if s.has_fe}
+    gsse = {s.step: s.rmse for s in gs if s.has_sse}
+    assert sorted(gsse) == sorted(rsse)
+    assert set(rfe) == set(gfe)
+    ftol = 1e-11 if prec == XPFC_FP64 else 1e-7
+    for st, r in rfe.items():
+        assert abs(gfe[st].F_ref - r.value) <= ftol * abs(r.value), (st, gfe[st].F_ref, r.value)
+        assert abs(gfe[st].mean - r.mean) <= (1e-14 if prec == XPFC_FP64 else 1e-9)
+    for st, r in rsse.items():
+        assert abs(gsse[st] - r) <= (1e-9 if prec == XPFC_FP64 else 1e-4) * abs(r) + 1e-300, (st, gsse[st], r)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_c1_config_1000_steps(prec):
+    """BASELINE 2-D square lattice, 256², FP64 plain (and FP32-mixed), 1000 steps."""
+    _flush("c1 prec=%d" % prec)
+    op, lp = make((256, 256), sigma=0.1, prec=prec, t_sample=100)
+    psi0 = O.noise_ic(op, 0.01, 0)
+    ref, rs = O.run(psi0, op, 1000, sample=True)
+    with Context(lp) as ctx:
+        ctx.set_field(psi0)
+        ctx.step(1000)
+        got = ctx.get_field()
+        gs = ctx.samples()
+    assert np.max(np.abs(got - ref)) <= (1e-12 if prec == XPFC_FP64 else 1e-6)
+    gfe = {s.step: s.F_ref for s in gs if s.has_fe}
+    for s in rs:
+        if s.kind == "FE":
+            assert abs(gfe[s.step] - s.value) <= (1e-12 if prec == XPFC_FP64 else 1e-7) * abs(s.value)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_operator_kernels_vs_oracle(prec):
+    op, lp = growth_case((16, 32), prec=prec)
+    rng = np.random.default_rng(11)
+    psi = 0.05 + 0.02 * rng.standard_normal(op.n)
+    h = op.n[-1] // 2 + 1
+    k2, c2 = O.k_tables(op)
+    k2, c2 = k2[..., :h], c2[..., :h]
+    psi_k = np.fft.rfftn(psi)
+    p2_k = np.fft.rfftn(O.power_function(psi, 2))
+    p3_k = np.fft.rfftn(O.power_function(psi, 3))
+    kp1 = O.euler_step(psi_k, p2_k, p3_k, c2, k2, op)
+    rtol = 1e-13 if prec == XPFC_FP64 else 3e-6
+    with Context(lp) as ctx:
+        for p in (2, 3, 4):
+            got = ctx.power_functions(psi, p)
+            assert np.allclose(got, O.power_function(psi, p), rtol=rtol, atol=0)
+        got = ctx.convolve_fields(psi_k)
+        assert np.max(np.abs(got - O.convolve_fields(psi_k, c2))) <= rtol * np.max(np.abs(psi_k))
+        got = ctx.euler_step(psi_k, p2_k, p3_k)
+        assert np.max(np.abs(got - kp1)) <= rtol * np.max(np.abs(kp1)) * 10
+        got = ctx.residual_calc(kp1, psi_k, p2_k, p3_k)
+        ref = O.residual_calc(kp1, psi_k, p2_k, p3_k, c2, k2, op)
+        assert np.max(np.abs(got - ref)) <= rtol * np.max(np.abs(ref)) * 100
+        got = ctx.scaling_step(psi)
+        assert np.allclose(got, O.scaling_step(psi, psi.size), rtol=rtol, atol=0)
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_fe_calc_matches_oracle(prec):
+    op, lp = growth_case((32, 16, 16), prec=prec)
+    psi = O.noise_ic(op, 0.02, 5)
+    F_ref, fm = O.free_energy(psi, op, O.k_tables(op)[1])
+    with Context(lp) as ctx:
+        ctx.set_field(psi)
+        F, f = ctx.fe_calc()
+    tol = 1e-12 if prec == XPFC_FP64 else 1e-7
+    assert abs(F - F_ref) <= tol * abs(F_ref)
+    assert abs(f - fm) <= tol * abs(fm)
+
+
+def test_mass_conserved_fp32():
+    _, lp = growth_case((32, 32, 32), prec=XPFC_FP32_MIXED, t_sample=0)
+    with Context(lp) as ctx:
+        ctx.initialize_noise(0.01, 4)
+        m0 = ctx.avg_calc()
+        ctx.step(200)
+        assert ctx.avg_calc() == m0
+        F, _ = ctx.fe_calc()
+        assert math.isfinite(F)

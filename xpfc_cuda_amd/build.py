@@ -1,0 +1,74 @@
+"""Build libxpfc.so (HIP kernels for gfx950 + C++ host runtime) in-tree.
+
+Usage: python -m xpfc_cuda_amd.build   (or xpfc_cuda_amd.build.build())
+Output: xpfc_cuda_amd/lib/libxpfc.so and xpfc_cuda_amd/bin/xpfc_run (CLI).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+from concurrent.futures import ThreadPoolExecutor
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "lib")
+BINDIR = os.path.join(HERE, "bin")
+OBJDIR = os.path.join(HERE, "_build")
+
+ARCH = os.environ.get("XPFC_OFFLOAD_ARCH", "gfx950")
+# packed FP32 instructions stay off (they are generated by -O3 otherwise)
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+            "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+            "-I", os.path.join(ROOT, "include")]
+
+SOURCES = ["xpfc_kernels.hip", "xpfc_host.cpp"]
+
+
+def _hipcc() -> str:
+    for c in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if c and os.path.exists(c):
+            return c
+    raise RuntimeError("hipcc not found")
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    out = "\n".join(l for l in r.stdout.splitlines() if "packed-fp32-ops" not in l
+                    and "is not a recognized feature" not in l)
+    if r.returncode != 0:
+        raise RuntimeError("command failed: %s\n%s" % (" ".join(cmd), out))
+    return out
+
+
+def build(verbose: bool = False) -> str:
+    hipcc = _hipcc()
+    os.makedirs(OBJDIR, exist_ok=True)
+    os.makedirs(LIBDIR, exist_ok=True)
+    os.makedirs(BINDIR, exist_ok=True)
+    objs = []
+    jobs = []
+    for s in SOURCES:
+        o = os.path.join(OBJDIR, s + ".o")
+        objs.append(o)
+        jobs.append([hipcc, *HIPFLAGS, "-c", os.path.join(CSRC, s), "-o", o])
+    with ThreadPoolExecutor(max_workers=len(jobs)) as ex:
+        for out in ex.map(_run, jobs):
+            if verbose and out.strip():
+                print(out)
+    lib = os.path.join(LIBDIR, "libxpfc.so")
+    tmp = lib + ".tmp"
+    _run([hipcc, *HIPFLAGS, "-shared", "-o", tmp, *objs])
+    os.replace(tmp, lib)
+    cli_src = os.path.join(CSRC, "xpfc_run.cpp")
+    if os.path.exists(cli_src):
+        cli = os.path.join(BINDIR, "xpfc_run")
+        _run([hipcc, *HIPFLAGS, cli_src, "-o", cli, "-L", LIBDIR, "-lxpfc",
+              "-Wl,-rpath,$ORIGIN/../lib"])
+    return lib
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

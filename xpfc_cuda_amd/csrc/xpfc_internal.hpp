@@ -1,0 +1,88 @@
+// Internal structures shared by the host runtime and the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstddef>
+#include <cstdint>
+
+namespace xpfc {
+
+// A family of equal-length lines visited by a strided FFT pass.  Line points
+// are `rs` elements apart; neighbouring lines (columns) are `cs` apart; a
+// batch of `ncols` columns repeats every `bs` elements.  `kind` tells the
+// k-space update whether the lines belong to the main half spectrum (kind 0,
+// column = kb*H + kz) or to the Nyquist plane kz = n2/2 (kind 1, column = kb).
+struct Seg {
+  long long off;
+  long long bs;
+  int rs, cs;
+  int ncols, nbatch, ntc;
+  int kind;
+};
+struct Segs {
+  Seg s[2];
+  int tiles0, tiles1;
+};
+
+// Wavenumber geometry of the packed spectrum.
+struct KGeom {
+  int n0, n1, H;
+  double dk0, dk1, dk2;  // 2*pi/L per axis
+};
+
+// Physics constants of the semi-implicit update (all FP64).
+struct Phys {
+  double dtM;        // dt * Mo
+  double one_m_eps;  // 1 - eps
+  double inv_dt;     // 1 / dt          (residual)
+  double Mres;       // residual mobility (Mo, or (int)Mo like the reference)
+  double kcut2;      // |k|^2 beyond which every DCF peak term is below 2^-72
+  int npk;
+  int dcf_max;       // 0 = sum of peaks, 1 = max over peaks
+  double pk[4], inv2a2[4], amp[4];
+};
+
+// Real-space pass constants.
+struct ZArgs {
+  double mean;   // FP64 mean density (the k = 0 mode)
+  double a, b;   // N(psi) = -a psi^2 + b psi^3
+  double scale;  // 1/N for the inverse transform
+};
+
+enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
+
+// Per-workgroup FP64 partial sums.  X pass: {sum w c2 |psi_k|^2, sum w |R_k|^2}.
+// Z pass: {sum psi, sum psi^2, sum psi^3, sum psi^4}.
+constexpr int XPART = 2;
+constexpr int ZPART = 4;
+constexpr int NSUMS = 6;
+
+// launch entry points (xpfc_kernels.hip)
+int launch_strided(int prec, void* buf, const Segs& segs, int L, int dir, const void* tw,
+                   hipStream_t s);
+int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, const void* tw,
+                 const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s);
+int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
+                 long long nlines, int H, const ZArgs& za, const void* tw, const void* twp,
+                 double* partial, bool sample, hipStream_t s);
+int zpass_blocks(int H, long long nlines);
+int strided_tiles(int prec, int L, const Segs& segs, Segs* out);
+int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s);
+int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
+                       const Phys& ph, double* partial, int* nblocks, hipStream_t s);
+
+// operator-level kernels on natural layouts (real [N], spectrum [n0][n1][n2/2+1])
+int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s);
+int launch_scale(int prec, void* buf, long long ncomplex_or_real, double f, hipStream_t s);
+int launch_convolve(int prec, const void* psi_k, void* out, const KGeom& g, const Phys& ph,
+                    int n2, hipStream_t s);
+int launch_euler(int prec, const void* psi_k, const void* p2_k, const void* p3_k, void* out,
+                 const KGeom& g, const Phys& ph, double a, double b, int n2, hipStream_t s);
+int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void* p2_k,
+                    const void* p3_k, void* out, const KGeom& g, const Phys& ph, double a,
+                    double b, int n2, hipStream_t s);
+int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s);
+int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s);
+int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s);
+int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s);
+
+}  // namespace xpfc
