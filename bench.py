@@ -63,7 +63,8 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
     number of steps of the same 512^3 workload; steps/s."""
     import numpy as np
     from oracle import xpfc_oracle as O
-    workers = os.cpu_count() or 1
+    # the host share of one GPU is 16 threads; never more than the machine has
+    workers = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
     k1 = 2.0 * math.pi
     k2 = k1 * 2.0 / math.sqrt(3.0)
     L = n * 0.125
@@ -71,12 +72,12 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
                   peaks=[O.Peak(k1, 1.0, 1.0), O.Peak(k2, 1.0, 1.0)], dcf_mode="max",
                   t_sample=100)
     psi = op.psi0 + 0.01 * np.random.default_rng(2).standard_normal(op.n)
-    # one step to warm caches/plans, then time up to nsteps_cap steps within the budget
-    O.run(psi, op, 1, sample=False, workers=workers)
+    st = O.Stepper(psi, op, workers)   # tables built outside the timed window
+    st.step()                          # warm plans and pages
     done = 0
     t0 = time.perf_counter()
     while done < nsteps_cap and time.perf_counter() - t0 < budget_s:
-        psi, _ = O.run(psi, op, 1, sample=False, workers=workers)
+        st.step()
         done += 1
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "steps/s", "cores": workers, "kind": "port",
@@ -92,7 +93,21 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run --steps unsampled steps and one field read-back (its D2D copy of "
+                         "the packed spectrum is the known-size copy for counter calibration)")
     args = ap.parse_args()
+    if args.traffic_probe:
+        import xpfc_cuda_amd as X
+        p = fcc_params(args.n, 1, t_sample=0)
+        with X.Context(p) as ctx:
+            ctx.initialize_noise(0.01, 2)
+            ctx.step(args.steps)
+            ctx.get_field()
+        print(json.dumps({"traffic_probe": True, "n": args.n, "steps": args.steps,
+                          "copy_bytes": spectrum_bytes(args.n, 1),
+                          "xstep_algorithmic_bytes": 4 * spectrum_bytes(args.n, 1)}), flush=True)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the workload would shard as x-slabs with two all-to-all transposes per step)",

@@ -299,6 +299,36 @@ class Sample:
     mean: float = float("nan")
 
 
+class Stepper:
+    """The real-transform form of the loop body of main.cu:309-408 with the
+    k-space tables built once: psi_k' = euler_step(psi_k, FFT P2, FFT P3),
+    psi = IFFT(psi_k')/M2, psi_k = FFT(psi)."""
+
+    def __init__(self, psi_real: np.ndarray, p: Params, workers: int = 1):
+        self.p = p
+        self.workers = workers
+        k2, c2 = k_tables(p)
+        h = p.n[-1] // 2 + 1
+        self.c2_full = c2
+        self.k2 = np.ascontiguousarray(k2[..., :h])
+        self.c2 = np.ascontiguousarray(c2[..., :h])
+        self.psi = np.array(psi_real, dtype=np.float64)
+        self.psi_k = _rfftn(self.psi, workers)
+
+    def step(self, want_sse: bool = False):
+        p, w = self.p, self.workers
+        p2_k = _rfftn(power_function(self.psi, 2, p.powf_fp32), w)
+        p3_k = _rfftn(power_function(self.psi, 3, p.powf_fp32), w)
+        psi_kp1 = euler_step(self.psi_k, p2_k, p3_k, self.c2, self.k2, p)
+        rmse = None
+        if want_sse:
+            res = residual_calc(psi_kp1, self.psi_k, p2_k, p3_k, self.c2, self.k2, p)
+            rmse = rmse_from_residual(_full_from_half(res, p.n, w), p, w)
+        self.psi = _irfftn(psi_kp1, tuple(p.n), w)
+        self.psi_k = _rfftn(self.psi, w)
+        return rmse
+
+
 def run(psi0_real: np.ndarray, p: Params, nsteps: int, sample: bool = True,
         workers: int = 1, literal: bool = False):
     """The time loop of main.cu:286-455 on one device.
@@ -314,53 +344,47 @@ def run(psi0_real: np.ndarray, p: Params, nsteps: int, sample: bool = True,
     and its spectra are therefore Hermitian (pinned by a test).
     Returns (psi_final_real, samples)."""
     assert psi0_real.shape == tuple(p.n)
-    k2, c2 = k_tables(p)
     M2 = p.ncells
-    psi = np.array(psi0_real, dtype=np.float64)
     samples: List[Sample] = []
+    if not literal:
+        st = Stepper(psi0_real, p, workers)
+        if sample:
+            F, fm = free_energy(st.psi, p, st.c2_full, workers)
+            samples.append(Sample(0, "FE", F, fm, mean(st.psi)))
+        for n in range(nsteps):
+            rmse = st.step(want_sse=sample and n % p.t_sample == 0)
+            if rmse is not None:
+                samples.append(Sample(n, "SSE", rmse))
+            if sample and (n + 1) % p.t_sample == 0:
+                F, fm = free_energy(st.psi, p, st.c2_full, workers)
+                samples.append(Sample(n + 1, "FE", F, fm, mean(st.psi)))
+        if sample:
+            F, fm = free_energy(st.psi, p, st.c2_full, workers)
+            samples.append(Sample(nsteps, "FE", F, fm, mean(st.psi)))
+        return st.psi, samples
+    k2, c2 = k_tables(p)
+    psi = np.array(psi0_real, dtype=np.float64)
     if sample:
         F, fm = free_energy(psi, p, c2, workers)
         samples.append(Sample(0, "FE", F, fm, mean(psi)))
-    if literal:
-        psi_k = _fftn(psi.astype(np.complex128), workers)
-    else:
-        h = p.n[-1] // 2 + 1
-        k2 = k2[..., :h]
-        c2 = c2[..., :h]
-        psi_k = _rfftn(psi, workers)
-    n = 0
+    # literal restatement: full complex transforms as the reference
+    psi_k = _fftn(psi.astype(np.complex128), workers)
     for n in range(nsteps):
-        p2 = power_function(psi, 2, p.powf_fp32)
-        p3 = power_function(psi, 3, p.powf_fp32)
-        if literal:
-            p2_k = _fftn(p2.astype(np.complex128), workers)
-            p3_k = _fftn(p3.astype(np.complex128), workers)
-        else:
-            p2_k = _rfftn(p2, workers)
-            p3_k = _rfftn(p3, workers)
+        p2_k = _fftn(power_function(psi, 2, p.powf_fp32).astype(np.complex128), workers)
+        p3_k = _fftn(power_function(psi, 3, p.powf_fp32).astype(np.complex128), workers)
         psi_kp1 = euler_step(psi_k, p2_k, p3_k, c2, k2, p)
         if sample and n % p.t_sample == 0:
             res = residual_calc(psi_kp1, psi_k, p2_k, p3_k, c2, k2, p)
-            if not literal:
-                res = _full_from_half(res, p.n, workers)
             samples.append(Sample(n, "SSE", rmse_from_residual(res, p, workers)))
-        if literal:
-            psi = np.real(scaling_step(_ifftn(psi_kp1, workers) * M2, M2))
-            psi_k = _fftn(psi.astype(np.complex128), workers)
-        else:
-            psi = _irfftn(psi_kp1, tuple(p.n), workers)
-            psi_k = _rfftn(psi, workers)
+        psi = np.real(scaling_step(_ifftn(psi_kp1, workers) * M2, M2))   # copy_matrix: Re
+        psi_k = _fftn(psi.astype(np.complex128), workers)
         if sample and (n + 1) % p.t_sample == 0:
-            F, fm = free_energy(psi, p, _full_c2(p), workers)
+            F, fm = free_energy(psi, p, c2, workers)
             samples.append(Sample(n + 1, "FE", F, fm, mean(psi)))
     if sample:
-        F, fm = free_energy(psi, p, _full_c2(p), workers)
+        F, fm = free_energy(psi, p, c2, workers)
         samples.append(Sample(nsteps, "FE", F, fm, mean(psi)))
     return psi, samples
-
-
-def _full_c2(p: Params) -> np.ndarray:
-    return k_tables(p)[1]
 
 
 def _full_from_half(res_half: np.ndarray, n: Sequence[int], workers: int) -> np.ndarray:
