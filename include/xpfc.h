@@ -138,6 +138,16 @@ int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
  * pass, the real-space pass and the axis-1 forward pass (0 in 2-D), ms[4] =
  * average ms per whole step. */
 int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
+/* Pass variants: v[0] = tile width (columns per workgroup, 4/8/16) of the
+ * axis-0 pass, v[1] = of the axis-1 passes, v[2] = workgroup size (64..512)
+ * of the real-space pass.  Variants regroup lines into workgroups and never
+ * change the computed field.  xpfc_autotune times every variant on scratch
+ * buffers, keeps the fastest (cached per precision and grid for the process)
+ * and returns the choice in v (may be NULL); xpfc_step runs it on first use
+ * unless XPFC_AUTOTUNE=0 or xpfc_set_variants was called (0 = keep). */
+int xpfc_autotune(xpfc_ctx* ctx, int* v);
+int xpfc_set_variants(xpfc_ctx* ctx, const int* v);
+int xpfc_get_variants(const xpfc_ctx* ctx, int* v);
 /* Block until all work of the context has finished. */
 int xpfc_sync(xpfc_ctx* ctx);
 

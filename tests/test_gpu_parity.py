@@ -167,3 +167,32 @@ def test_mass_conserved_fp32():
         assert ctx.avg_calc() == m0
         F, _ = ctx.fe_calc()
         assert math.isfinite(F)
+
+
+@pytest.mark.parametrize("shape", [(64, 64), (32, 64, 32)])
+@pytest.mark.parametrize("prec", PRECS)
+def test_variants_give_bitwise_identical_fields(shape, prec):
+    """Tile width / block size only regroup lines: the field must not change."""
+    op, lp = growth_case(shape, prec=prec, t_sample=0)
+    psi0 = O.noise_ic(op, 0.01, 9)
+    outs = []
+    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512)]:
+        with Context(lp) as ctx:
+            try:
+                ctx.set_variants(*v)
+            except Exception:
+                continue
+            ctx.set_field(psi0)
+            ctx.step(12)
+            outs.append((ctx.variants(), ctx.get_field()))
+    assert len(outs) >= 3
+    for v, o in outs[1:]:
+        assert np.array_equal(o, outs[0][1]), (v, outs[0][0])
+
+
+def test_autotune_picks_available_variant():
+    _, lp = make((64, 64, 64), prec=XPFC_FP32_MIXED)
+    with Context(lp) as ctx:
+        v = ctx.autotune()
+        assert v[0] in (4, 8, 16) and v[1] in (4, 8, 16) and v[2] in (64, 128, 256, 512)
+        assert ctx.variants() == v

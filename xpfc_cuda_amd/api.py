@@ -89,6 +89,9 @@ def load_library(path: Optional[str] = None):
         "xpfc_time_steps": (i, [vp, i, P(ct.c_float)]),
         "xpfc_sync": (i, [vp]),
         "xpfc_time_passes": (i, [vp, i, P(d)]),
+        "xpfc_autotune": (i, [vp, P(i)]),
+        "xpfc_set_variants": (i, [vp, P(i)]),
+        "xpfc_get_variants": (i, [vp, P(i)]),
         "xpfc_fe_calc": (i, [vp, P(d), P(d)]),
         "xpfc_avg_calc": (i, [vp, P(d)]),
         "xpfc_sse_calc": (i, [vp, P(d)]),
@@ -362,6 +365,20 @@ class Context:
         ms = (ct.c_double * 5)()
         _check(lib().xpfc_time_passes(self.handle, int(nsteps), ms), "xpfc_time_passes")
         return list(ms)
+
+    def autotune(self) -> Tuple[int, int, int]:
+        v = (ct.c_int * 3)()
+        _check(lib().xpfc_autotune(self.handle, v), "xpfc_autotune")
+        return tuple(v)
+
+    def set_variants(self, tc_x: int = 0, tc_y: int = 0, ntz: int = 0):
+        v = (ct.c_int * 3)(tc_x, tc_y, ntz)
+        _check(lib().xpfc_set_variants(self.handle, v), "xpfc_set_variants")
+
+    def variants(self) -> Tuple[int, int, int]:
+        v = (ct.c_int * 3)()
+        _check(lib().xpfc_get_variants(self.handle, v), "xpfc_get_variants")
+        return tuple(v)
 
     def sync(self):
         _check(lib().xpfc_sync(self.handle), "xpfc_sync")

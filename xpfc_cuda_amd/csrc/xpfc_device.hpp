@@ -1,0 +1,53 @@
+// Device helpers shared by the pass kernels and the operator kernels.
+#pragma once
+#include "fft_core.hpp"
+#include "xpfc_internal.hpp"
+
+namespace xpfc {
+
+// DCF (initialize.cu:209-217): sum or max over peaks of
+// amp_i * exp(-(|k| - k_i)^2 / (2 alpha_i^2)), amp_i = exp(-(sigma k_i)^2/(2 rho_i B_i)).
+// Terms with exponent >= 50 (below 2e-22 relative) are skipped.
+__device__ __forceinline__ double c2hat(double kmag, const Phys& ph) {
+  double r = 0.0;
+  for (int i = 0; i < ph.npk; ++i) {
+    const double d = kmag - ph.pk[i];
+    const double e = d * d * ph.inv2a2[i];
+    const double term = (e < 50.0) ? ph.amp[i] * exp(-e) : 0.0;
+    r = ph.dcf_max ? fmax(r, term) : r + term;
+  }
+  return r;
+}
+
+// FFT_freq.cu:17-24 index -> signed wavenumber index (Nyquist positive)
+__device__ __forceinline__ int signed_index(int i, int n) { return i <= n / 2 ? i : i - n; }
+
+// Block reduction of K FP64 values in a fixed order (deterministic):
+// wave64 butterfly with __shfl_xor, then one LDS slot per wave.
+template <int NT, int K>
+__device__ __forceinline__ void block_sum(double (&a)[K], double* __restrict__ out) {
+  constexpr int NW = (NT + 63) / 64;
+  __shared__ double red[NW][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1)
+      if (off < NT) a[k] += __shfl_xor(a[k], off, 64);
+  }
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  if (lane == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) red[w][k] = a[k];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      double s = 0.0;
+      for (int i = 0; i < NW; ++i) s += red[i][k];
+      out[k] = s;
+    }
+  }
+}
+
+}  // namespace xpfc

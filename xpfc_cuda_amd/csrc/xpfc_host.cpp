@@ -7,6 +7,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -44,7 +47,7 @@ int fail(int code, const char* fmt, ...) {
 #define LCHK(expr)                                                                    \
   do {                                                                                \
     int r_ = (expr);                                                                  \
-    if (r_ != 0) {                                                                    \
+    if (r_ < 0) {                                                                     \
       hipError_t e_ = hipGetLastError();                                              \
       return fail(XPFC_ERR_DEVICE, "%s failed (%d): %s", #expr, r_,                   \
                   hipGetErrorString(e_));                                             \
@@ -73,7 +76,9 @@ struct xpfc_ctx {
   double* px = nullptr;  // X-pass partials
   double* pz = nullptr;  // Z-pass partials
   double* sums = nullptr;  // [SUM_SLOTS][NSUMS]
-  int npx_cap = 0, npz = 0, npx_step = 0;
+  int npx_cap = 0, npz_cap = 0;
+  int tc_x = 0, tc_y = 0, ntz = 256;  // pass variants (tile width, real-space block)
+  bool tuned = false;
   hipStream_t stream = nullptr;
   double mean = 0.0;
   KGeom g{};
@@ -323,6 +328,14 @@ double c2hat_host(double kmag, const xpfc_params& p) {
   return r;
 }
 
+// widest valid tile (16 columns FP32 / 8 FP64 keeps every row access 128 bytes)
+int default_tc(int prec, int L, const Segs& s) {
+  const int pref = prec == 1 ? 16 : 8;
+  for (int tc = pref; tc >= 4; tc /= 2)
+    if (strided_blocks(prec, L, tc, s) >= 0) return tc;
+  return -1;
+}
+
 void free_all(xpfc_ctx* c) {
   void* ptrs[] = {c->psi, c->work, c->aux, c->tw0, c->tw1, c->twz, c->twp,
                   c->px, c->pz, c->sums};
@@ -398,14 +411,20 @@ extern "C" int xpfc_create(const xpfc_params* pin, xpfc_ctx** out) {
   Segs sy{};
   sy.s[0] = Seg{0, (long long)c->n1 * c->H, c->H, 1, c->H, c->n0, 0, 0};
   sy.s[1] = Seg{c->nmain, 0, 1, c->n1, c->n0, 1, 0, 1};
-  if (strided_tiles(c->prec, c->n0, sx, &c->segX) != 0) return bail(fail(XPFC_ERR_ARG, "unsupported n0"));
-  if (c->n1 > 1 && strided_tiles(c->prec, c->n1, sy, &c->segY) != 0)
-    return bail(fail(XPFC_ERR_ARG, "unsupported n1"));
-  c->npx_step = c->segX.tiles0 + c->segX.tiles1;
-  c->npx_cap = std::max(c->npx_step, 4096);
-  c->npz = zpass_blocks(c->H, (long long)c->n0 * c->n1);
+  c->segX = sx;
+  c->segY = sy;
+  // default variants: the widest tile that fits, 256-thread real-space blocks
+  c->tc_x = default_tc(c->prec, c->n0, sx);
+  c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, sy) : 0;
+  if (c->tc_x <= 0 || (c->n1 > 1 && c->tc_y <= 0)) return bail(fail(XPFC_ERR_ARG, "unsupported grid"));
+  c->ntz = zpass_blocks(c->H, 256, 1) > 0 ? 256 : 512;
+  // partial buffers sized for the largest variant (narrowest tile, smallest block)
+  c->npx_cap = 4096;
+  for (int tc : {4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, sx));
+  c->npz_cap = 1;
+  for (int nz : {64, 128, 256, 512}) c->npz_cap = std::max(c->npz_cap, zpass_blocks(c->H, nz, (long long)c->n0 * c->n1));
   if (hipMalloc((void**)&c->px, (size_t)c->npx_cap * XPART * sizeof(double)) != hipSuccess ||
-      hipMalloc((void**)&c->pz, (size_t)c->npz * ZPART * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->pz, (size_t)c->npz_cap * ZPART * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->sums, (size_t)SUM_SLOTS * NSUMS * sizeof(double)) != hipSuccess)
     return bail(fail(XPFC_ERR_NOMEM, "partial-sum allocation failed"));
 
@@ -489,14 +508,14 @@ int upload_fluct(xpfc_ctx* c, const double* host, double mean) {
 
 // full forward transform of a spectrum-layout buffer whose z axis is already done
 int forward_xy(xpfc_ctx* c, void* spec, bool do_x) {
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, -1, c->tw1, c->stream));
-  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, -1, c->tw0, c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, c->stream));
+  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec) {
-  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, +1, c->tw0, c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, +1, c->tw1, c->stream));
+  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, c->stream));
   return XPFC_OK;
 }
 
@@ -507,7 +526,7 @@ int psi_to_real(xpfc_ctx* c, bool sums) {
   int rc = inverse_xy(c, c->aux);
   if (rc) return rc;
   LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), c->aux, nullptr,
-                    (long long)c->n0 * c->n1, c->H, c->za, c->twz, c->twp, c->pz, sums, c->stream));
+                    (long long)c->n0 * c->n1, c->H, 256, c->za, c->twz, c->twp, c->pz, sums, c->stream));
   return XPFC_OK;
 }
 
@@ -527,12 +546,12 @@ extern "C" int xpfc_set_field(xpfc_ctx* c, const double* host) {
   const long long nl = (long long)c->n0 * c->n1;
   // psi_k = FFT(psi - mean); the k = 0 slot is then zero by construction
   LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, c->aux, nullptr, c->psi, ny_of(c, c->psi), nl, c->H,
-                    c->za, c->twz, c->twp, nullptr, false, c->stream));
+                    256, c->za, c->twz, c->twp, nullptr, false, c->stream));
   if ((rc = forward_xy(c, c->psi, true))) return rc;
   HIPCHK(hipMemsetAsync(c->psi, 0, c->csize, c->stream));
   // work = FFT_yz(N(psi)): the nonlinear term, all axes but axis 0
   LCHK(launch_zpass(c->prec, Z_R2C_NL, c->aux, nullptr, c->work, ny_of(c, c->work), nl, c->H,
-                    c->za, c->twz, c->twp, nullptr, false, c->stream));
+                    256, c->za, c->twz, c->twp, nullptr, false, c->stream));
   if ((rc = forward_xy(c, c->work, false))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->have_field = true;
@@ -660,15 +679,23 @@ bool sample_step(const xpfc_ctx* c, long long n) {
   return ts > 0 && (n % ts == 0 || (n + 1) % ts == 0);
 }
 
-int launch_one_step(xpfc_ctx* c, bool sample, double* sums_out) {
+// The four launches of one step (two in 2-D).  With `ev`, an event is recorded
+// after each pass (ev[0..3]) for per-pass timing.
+int launch_one_step(xpfc_ctx* c, bool sample, double* sums_out, hipEvent_t* ev = nullptr) {
   const long long nl = (long long)c->n0 * c->n1;
-  LCHK(launch_xstep(c->prec, c->work, c->psi, c->segX, c->n0, c->tw0, c->g, c->ph, c->px, sample,
-                    c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, +1, c->tw1, c->stream));
+  const int nbx = launch_xstep(c->prec, c->work, c->psi, c->segX, c->n0, c->tc_x, c->tw0, c->g,
+                               c->ph, c->px, sample, c->stream);
+  LCHK(nbx);
+  if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, +1, c->tw1, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
   LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl,
-                    c->H, c->za, c->twz, c->twp, c->pz, sample, c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, -1, c->tw1, c->stream));
-  if (sample) LCHK(launch_finalize(c->px, c->npx_step, c->pz, c->npz, sums_out, c->stream));
+                    c->H, c->ntz, c->za, c->twz, c->twp, c->pz, sample, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  if (sample)
+    LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->H, c->ntz, nl), sums_out, c->stream));
   return XPFC_OK;
 }
 
@@ -741,10 +768,142 @@ int fe_now(xpfc_ctx* c, double* F, double* fm, double* mean_rs);
 
 }  // namespace
 
+// ---------------------------------------------------------------------------
+// autotuner: times every variant of each pass on scratch buffers and keeps the
+// fastest; results are cached per (precision, grid) for the process.  The
+// variants only regroup lines into workgroups, so the choice never changes
+// the computed field.
+// ---------------------------------------------------------------------------
+namespace {
+std::mutex g_tune_mu;
+std::map<std::array<int, 4>, std::array<int, 3>> g_tune_cache;
+
+int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int reps, float* ms) {
+  const long long nl = (long long)c->n0 * c->n1;
+  auto launch = [&]() -> int {
+    if (which == 0)
+      return launch_xstep(c->prec, a, b, c->segX, c->n0, v, c->tw0, c->g, c->ph, c->px, false, c->stream);
+    if (which == 1) return launch_strided(c->prec, a, c->segY, c->n1, v, -1, c->tw1, c->stream);
+    return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, v, c->za, c->twz,
+                        c->twp, c->pz, false, c->stream);
+  };
+  int r = launch();  // warm-up; also tells whether the variant exists
+  if (r == -3) return -3;
+  if (r < 0) return fail(XPFC_ERR_DEVICE, "autotune launch failed (%d)", r);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i)
+    if (launch() < 0) return fail(XPFC_ERR_DEVICE, "autotune launch failed");
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  *ms /= (float)reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int autotune(xpfc_ctx* c) {
+  const std::array<int, 4> key{c->prec, c->n0, c->n1, c->n2};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      c->tc_x = it->second[0];
+      c->tc_y = it->second[1];
+      c->ntz = it->second[2];
+      c->tuned = true;
+      return XPFC_OK;
+    }
+  }
+  const size_t sbytes = (size_t)(c->nmain + c->nny) * c->csize;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, sbytes) != hipSuccess || hipMalloc(&b, sbytes) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return fail(XPFC_ERR_NOMEM, "autotune scratch allocation failed");
+  }
+  int rc = XPFC_OK;
+  HIPCHK(hipMemsetAsync(a, 0, sbytes, c->stream));
+  HIPCHK(hipMemsetAsync(b, 0, sbytes, c->stream));
+  const int reps = c->N >= (1LL << 24) ? 4 : 16;
+  int best[3] = {c->tc_x, c->tc_y, c->ntz};
+  const std::vector<int> tcs{4, 8, 16}, ntzs{64, 128, 256, 512};
+  for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
+    if (which == 1 && c->n1 <= 1) continue;
+    float bt = 1e30f;
+    for (int v : (which == 2 ? ntzs : tcs)) {
+      float ms = 0.f;
+      const int r = time_pass(c, which, v, a, b, reps, &ms);
+      if (r == -3) continue;
+      if (r) { rc = r; break; }
+      if (ms < bt) { bt = ms; best[which] = v; }
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return rc;
+  c->tc_x = best[0];
+  c->tc_y = best[1];
+  c->ntz = best[2];
+  c->tuned = true;
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = {best[0], best[1], best[2]};
+  return XPFC_OK;
+}
+
+bool autotune_enabled() {
+  const char* e = std::getenv("XPFC_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+}  // namespace
+
+extern "C" int xpfc_autotune(xpfc_ctx* c, int* variants) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  int rc = autotune(c);
+  if (rc) return rc;
+  if (variants) {
+    variants[0] = c->tc_x;
+    variants[1] = c->tc_y;
+    variants[2] = c->ntz;
+  }
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
+  if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  const long long nl = (long long)c->n0 * c->n1;
+  if (v[0] > 0 && strided_blocks(c->prec, c->n0, v[0], c->segX) < 0)
+    return fail(XPFC_ERR_ARG, "tile width %d not available for n0 = %d", v[0], c->n0);
+  if (c->n1 > 1 && v[1] > 0 && strided_blocks(c->prec, c->n1, v[1], c->segY) < 0)
+    return fail(XPFC_ERR_ARG, "tile width %d not available for n1 = %d", v[1], c->n1);
+  if (v[2] > 0 && zpass_blocks(c->H, v[2], nl) < 0)
+    return fail(XPFC_ERR_ARG, "real-space block %d not available for n2 = %d", v[2], c->n2);
+  if (v[0] > 0) c->tc_x = v[0];
+  if (c->n1 > 1 && v[1] > 0) c->tc_y = v[1];
+  if (v[2] > 0) c->ntz = v[2];
+  c->tuned = true;  // explicit choice: the autotuner leaves it alone
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_get_variants(const xpfc_ctx* c, int* v) {
+  if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  v[0] = c->tc_x;
+  v[1] = c->tc_y;
+  v[2] = c->ntz;
+  return XPFC_OK;
+}
+
 extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!c->have_field) return fail(XPFC_ERR_STATE, "no field set");
   if (nsteps < 0) return fail(XPFC_ERR_ARG, "nsteps < 0");
+  if (!c->tuned && autotune_enabled()) {
+    int rc = autotune(c);
+    if (rc) return rc;
+  }
   if (c->step == 0 && c->p.t_sample > 0 && c->samples.empty()) {
     // main.cu:286: FE of the initial field
     double F, fm, mrs;
@@ -797,22 +956,16 @@ extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
 extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
   if (!c || !ms || nsteps <= 0) return fail(XPFC_ERR_ARG, "bad arguments");
   if (!c->have_field) return fail(XPFC_ERR_STATE, "no field set");
-  const long long nl = (long long)c->n0 * c->n1;
+  if (!c->tuned && autotune_enabled()) {
+    int rc = autotune(c);
+    if (rc) return rc;
+  }
   std::vector<hipEvent_t> ev((size_t)nsteps * 4 + 1);
   for (auto& e : ev) HIPCHK(hipEventCreate(&e));
   HIPCHK(hipEventRecord(ev[0], c->stream));
   for (int i = 0; i < nsteps; ++i) {
-    hipEvent_t* e = &ev[(size_t)i * 4];
-    LCHK(launch_xstep(c->prec, c->work, c->psi, c->segX, c->n0, c->tw0, c->g, c->ph, c->px, false,
-                      c->stream));
-    HIPCHK(hipEventRecord(e[1], c->stream));
-    if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, +1, c->tw1, c->stream));
-    HIPCHK(hipEventRecord(e[2], c->stream));
-    LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl,
-                      c->H, c->za, c->twz, c->twp, c->pz, false, c->stream));
-    HIPCHK(hipEventRecord(e[3], c->stream));
-    if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, -1, c->tw1, c->stream));
-    HIPCHK(hipEventRecord(e[4], c->stream));
+    int rc = launch_one_step(c, false, nullptr, &ev[(size_t)i * 4 + 1]);
+    if (rc) return rc;
     c->step++;
   }
   HIPCHK(hipEventSynchronize(ev.back()));
@@ -858,7 +1011,8 @@ int fe_now(xpfc_ctx* c, double* F, double* fm, double* mean_rs) {
   int rc = psi_to_real(c, true);
   if (rc) return rc;
   double* slot = c->sums + (size_t)(SUM_SLOTS - 1) * NSUMS;
-  LCHK(launch_finalize(c->px, nb, c->pz, c->npz, slot, c->stream));
+  LCHK(launch_finalize(c->px, nb, c->pz, zpass_blocks(c->H, 256, (long long)c->n0 * c->n1), slot,
+                       c->stream));
   double h[NSUMS];
   HIPCHK(hipMemcpyAsync(h, slot, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -979,7 +1133,7 @@ extern "C" int xpfc_fft_r2c(xpfc_ctx* c, const void* real_in, void* spec_out) {
   const long long nl = (long long)c->n0 * c->n1;
   ZArgs za = c->za;
   za.mean = 0.0;
-  LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, real_in, nullptr, c->aux, ny_of(c, c->aux), nl, c->H, za,
+  LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, real_in, nullptr, c->aux, ny_of(c, c->aux), nl, c->H, 256, za,
                     c->twz, c->twp, nullptr, false, c->stream));
   int rc = forward_xy(c, c->aux, true);
   if (rc) return rc;
@@ -998,7 +1152,7 @@ extern "C" int xpfc_fft_c2r(xpfc_ctx* c, const void* spec_in, void* real_out) {
   ZArgs za = c->za;
   za.mean = 0.0;
   za.scale = 1.0;  // unnormalised, as cuFFT
-  LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), real_out, nullptr, nl, c->H, za,
+  LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), real_out, nullptr, nl, c->H, 256, za,
                     c->twz, c->twp, nullptr, false, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   return XPFC_OK;

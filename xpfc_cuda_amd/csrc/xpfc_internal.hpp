@@ -56,16 +56,20 @@ constexpr int XPART = 2;
 constexpr int ZPART = 4;
 constexpr int NSUMS = 6;
 
-// launch entry points (xpfc_kernels.hip)
-int launch_strided(int prec, void* buf, const Segs& segs, int L, int dir, const void* tw,
+// Launch entry points of the step passes (xpfc_passes.hip).  `tc` is the tile
+// width (4, 8 or 16 columns per workgroup) of a strided pass and `ntz` the
+// workgroup size (64..512) of the real-space pass; a combination a size does
+// not support returns -3.  launch_xstep returns its workgroup count (= number
+// of X partials) on success.
+int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
                    hipStream_t s);
-int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, const void* tw,
+int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
                  const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s);
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                 long long nlines, int H, const ZArgs& za, const void* tw, const void* twp,
+                 long long nlines, int H, int ntz, const ZArgs& za, const void* tw, const void* twp,
                  double* partial, bool sample, hipStream_t s);
-int zpass_blocks(int H, long long nlines);
-int strided_tiles(int prec, int L, const Segs& segs, Segs* out);
+int zpass_blocks(int H, int ntz, long long nlines);
+int strided_blocks(int prec, int L, int tc, const Segs& segs);
 int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s);
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
                        const Phys& ph, double* partial, int* nblocks, hipStream_t s);

@@ -1,0 +1,227 @@
+// Operator-level HIP kernels on natural layouts (real [N], spectrum
+// [n0][n1][n2/2+1]): the reference's per-operator kernels, for drop-in
+// parity of each operator.  The time step itself does not use them.
+#include "xpfc_device.hpp"
+
+namespace xpfc {
+
+// ---------------------------------------------------------------------------
+// operator-level kernels on natural layouts
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void k_power(const T* __restrict__ src, T* __restrict__ dst, long long n, int p) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const T x = src[i];
+    T r = (T)1;
+    for (int q = 0; q < p; ++q) r *= x;
+    dst[i] = r;
+  }
+}
+
+template <typename T>
+__global__ void k_scale(T* __restrict__ buf, long long n, T f) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    buf[i] *= f;
+}
+
+// natural spectrum index e -> k^2 and c2
+__device__ __forceinline__ void natural_k(long long e, int n2h1, const KGeom& g, const Phys& ph,
+                                          double& k2, double& c2) {
+  const int kz = (int)(e % n2h1);
+  const long long r = e / n2h1;
+  const int kb = (int)(r % g.n1);
+  const int ka = (int)(r / g.n1);
+  const double kx = g.dk0 * signed_index(ka, g.n0), ky = g.dk1 * signed_index(kb, g.n1),
+               kzv = g.dk2 * kz;
+  k2 = kx * kx + ky * ky + kzv * kzv;
+  c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+}
+
+template <typename T>
+__global__ void k_convolve(const cpx<T>* __restrict__ psi_k, cpx<T>* __restrict__ out, long long n,
+                           int n2h1, KGeom g, Phys ph) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double k2, c2;
+    natural_k(e, n2h1, g, ph, k2, c2);
+    const cpx<T> v = psi_k[e];
+    out[e] = mk<cpx<T>>((T)(c2 * v.x), (T)(c2 * v.y));  // convolve_fields.cu:41-42
+  }
+}
+
+template <typename T>
+__global__ void k_euler(const cpx<T>* __restrict__ psi_k, const cpx<T>* __restrict__ p2,
+                        const cpx<T>* __restrict__ p3, cpx<T>* __restrict__ out, long long n,
+                        int n2h1, KGeom g, Phys ph, double a, double b) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double k2, c2;
+    natural_k(e, n2h1, g, ph, k2, c2);
+    const cpx<T> s = psi_k[e], q2 = p2[e], q3 = p3[e];
+    // euler_step.cu:50-59
+    const double nr = s.x + a * k2 * ph.dtM * q2.x - b * k2 * ph.dtM * q3.x;
+    const double ni = s.y + a * k2 * ph.dtM * q2.y - b * k2 * ph.dtM * q3.y;
+    const double den = 1.0 + k2 * ph.dtM * ph.one_m_eps - k2 * ph.dtM * c2;
+    out[e] = mk<cpx<T>>((T)(nr / den), (T)(ni / den));
+  }
+}
+
+template <typename T>
+__global__ void k_residual(const cpx<T>* __restrict__ kp1, const cpx<T>* __restrict__ k0,
+                           const cpx<T>* __restrict__ p2, const cpx<T>* __restrict__ p3,
+                           cpx<T>* __restrict__ out, long long n, int n2h1, KGeom g, Phys ph,
+                           double a, double b) {
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+       e += (long long)gridDim.x * blockDim.x) {
+    double k2, c2;
+    natural_k(e, n2h1, g, ph, k2, c2);
+    const cpx<T> s1 = kp1[e], s0 = k0[e], q2 = p2[e], q3 = p3[e];
+    // sse_calculation.cu:133-140
+    const double dr = -ph.Mres * k2 * (s0.x - a * q2.x + b * q3.x - c2 * s0.x);
+    const double di = -ph.Mres * k2 * (s0.y - a * q2.y + b * q3.y - c2 * s0.y);
+    out[e] = mk<cpx<T>>((T)((s1.x - s0.x) * ph.inv_dt - dr), (T)((s1.y - s0.y) * ph.inv_dt - di));
+  }
+}
+
+template <typename T>
+__global__ void k_pack(const cpx<T>* __restrict__ nat, cpx<T>* __restrict__ mainny, int n0n1, int H) {
+  const long long tot = (long long)n0n1 * (H + 1);
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % (H + 1));
+    const long long l = e / (H + 1);
+    if (kz < H) mainny[l * H + kz] = nat[e];
+    else mainny[(long long)n0n1 * H + l] = nat[e];
+  }
+}
+
+template <typename T>
+__global__ void k_unpack(const cpx<T>* __restrict__ mainny, cpx<T>* __restrict__ nat, int n0n1, int H) {
+  const long long tot = (long long)n0n1 * (H + 1);
+  for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * blockDim.x) {
+    const int kz = (int)(e % (H + 1));
+    const long long l = e / (H + 1);
+    nat[e] = kz < H ? mainny[l * H + kz] : mainny[(long long)n0n1 * H + l];
+  }
+}
+
+template <typename S, typename D>
+__global__ void k_convert(const S* __restrict__ src, D* __restrict__ dst, long long n) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    dst[i] = (D)src[i];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline int grid_for(long long n, int bs) {
+  long long g = (n + bs - 1) / bs;
+  if (g > 65536) g = 65536;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s) {
+  if (prec == 1)
+    hipLaunchKernelGGL(k_power<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, (float*)dst, n, p);
+  else
+    hipLaunchKernelGGL(k_power<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, (double*)dst, n, p);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_scale(int prec, void* buf, long long n, double f, hipStream_t s) {
+  if (prec == 1)
+    hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (float*)buf, n, (float)f);
+  else
+    hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (double*)buf, n, f);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+static long long natural_count(const KGeom& g, int n2) {
+  return (long long)g.n0 * g.n1 * (n2 / 2 + 1);
+}
+
+int launch_convolve(int prec, const void* psi_k, void* out, const KGeom& g, const Phys& ph, int n2,
+                    hipStream_t s) {
+  const long long n = natural_count(g, n2);
+  if (prec == 1)
+    hipLaunchKernelGGL(k_convolve<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<float>*)psi_k, (cpx<float>*)out, n, n2 / 2 + 1, g, ph);
+  else
+    hipLaunchKernelGGL(k_convolve<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<double>*)psi_k, (cpx<double>*)out, n, n2 / 2 + 1, g, ph);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_euler(int prec, const void* psi_k, const void* p2_k, const void* p3_k, void* out,
+                 const KGeom& g, const Phys& ph, double a, double b, int n2, hipStream_t s) {
+  const long long n = natural_count(g, n2);
+  if (prec == 1)
+    hipLaunchKernelGGL(k_euler<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<float>*)psi_k, (const cpx<float>*)p2_k, (const cpx<float>*)p3_k,
+                       (cpx<float>*)out, n, n2 / 2 + 1, g, ph, a, b);
+  else
+    hipLaunchKernelGGL(k_euler<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<double>*)psi_k, (const cpx<double>*)p2_k, (const cpx<double>*)p3_k,
+                       (cpx<double>*)out, n, n2 / 2 + 1, g, ph, a, b);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void* p2_k,
+                    const void* p3_k, void* out, const KGeom& g, const Phys& ph, double a,
+                    double b, int n2, hipStream_t s) {
+  const long long n = natural_count(g, n2);
+  if (prec == 1)
+    hipLaunchKernelGGL(k_residual<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<float>*)psi_kp1, (const cpx<float>*)psi_k, (const cpx<float>*)p2_k,
+                       (const cpx<float>*)p3_k, (cpx<float>*)out, n, n2 / 2 + 1, g, ph, a, b);
+  else
+    hipLaunchKernelGGL(k_residual<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<double>*)psi_kp1, (const cpx<double>*)psi_k, (const cpx<double>*)p2_k,
+                       (const cpx<double>*)p3_k, (cpx<double>*)out, n, n2 / 2 + 1, g, ph, a, b);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s) {
+  const long long n = (long long)n0n1 * (H + 1);
+  if (prec == 1)
+    hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<float>*)natural, (cpx<float>*)main_ny, n0n1, H);
+  else
+    hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<double>*)natural, (cpx<double>*)main_ny, n0n1, H);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s) {
+  const long long n = (long long)n0n1 * (H + 1);
+  if (prec == 1)
+    hipLaunchKernelGGL(k_unpack<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<float>*)main_ny, (cpx<float>*)natural, n0n1, H);
+  else
+    hipLaunchKernelGGL(k_unpack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
+                       (const cpx<double>*)main_ny, (cpx<double>*)natural, n0n1, H);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s) {
+  if (prec == 1)
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, dst, n);
+  else
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s) {
+  if (prec == 1)
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s, src, (float*)dst, n);
+  else
+    hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, src, (double*)dst, n);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace xpfc

@@ -1,0 +1,546 @@
+// HIP kernels of the XPFC time step for gfx950 (MI355X).
+//
+// Spectrum layout (packed half spectrum): for a real field of n0 x n1 x n2
+// points (n2 contiguous; 2-D fields use n1 = 1) the spectrum is
+//   main[n0][n1][H]  (kz = 0 .. H-1, H = n2/2)  followed by  ny[n0][n1] (kz = H).
+// The k = 0 slot of `main` always holds 0: the mean density (the k = 0 mode)
+// is carried separately in FP64 and added back in real space.
+//
+// One time step = four passes over HBM (two in 2-D):
+//   k_xstep   : lines along axis 0: forward FFT of the nonlinear term, the
+//               semi-implicit update of psi_k, inverse FFT      (euler_step.cu:50-59)
+//   k_strided : lines along axis 1, inverse                      (3-D only)
+//   k_zpass   : lines along axis 2: C2R, 1/N, N(psi) = -a psi^2 + b psi^3,
+//               R2C                                               (power_functions.cu:34-53,
+//                                                                  scaling_step.cu:48-49,
+//                                                                  copy_matrix.cu:44-45)
+//   k_strided : lines along axis 1, forward                      (3-D only)
+// The sampled diagnostics (FE_calc.cu:64, sse_calculation.cu:133-140,
+// Avg_calc.cu:3-18) are FP64 partial sums fused into k_xstep and k_zpass.
+//
+// Variants: the strided kernels take the tile width TC (columns per
+// workgroup) and the real-space kernel its workgroup size NTZ as template
+// parameters.  They change only how lines are grouped into workgroups, never
+// the arithmetic of a line, so every variant gives bitwise-identical fields;
+// the autotuner picks the fastest per size.
+#include "xpfc_device.hpp"
+
+namespace xpfc {
+
+constexpr int PADT = 1;  // LDS padding (complex elements) per tile row
+
+template <int L> constexpr int elems_per_thread() { return L < 8 ? L : 8; }
+template <int L, int TC> constexpr int tile_threads() { return TC * (L / elems_per_thread<L>()); }
+template <typename T, int L, int TC> constexpr bool tile_ok() {
+  return tile_threads<L, TC>() <= 1024 &&
+         (long long)L * (TC + PADT) * (long long)sizeof(cpx<T>) <= 144 * 1024;
+}
+
+// XCD-aware block order: the dispatcher deals consecutive workgroups to the 8
+// XCDs round-robin; remap so that each XCD receives a contiguous run of tiles
+// and neighbouring column tiles share one L2.
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  constexpr int X = 8;
+  const int per = n / X, rem = n % X;
+  const int xcd = b % X, slot = b / X;
+  return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
+}
+
+struct TileLoc {
+  const Seg* sg;
+  int col;
+  bool act;
+  long long off;
+};
+
+__device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, int TC) {
+  TileLoc r;
+  const bool second = tile >= segs.tiles0;
+  if (second) tile -= segs.tiles0;
+  r.sg = second ? &segs.s[1] : &segs.s[0];
+  const int b = tile / r.sg->ntc, ct = tile - b * r.sg->ntc;
+  r.col = ct * TC + ci;
+  r.act = r.col < r.sg->ncols;
+  r.off = r.sg->off + (long long)b * r.sg->bs + (long long)r.col * r.sg->cs;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// strided pass: forward (DIR=-1) or inverse (DIR=+1) transform of every line
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TC, int DIR>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
+void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw) {
+  using C = cpx<T>;
+  constexpr int E = elems_per_thread<L>();
+  __shared__ C lds[L * (TC + PADT)];
+  const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
+  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
+  const int rs = loc.sg->rs;
+  C* p = buf + loc.off;
+  C v[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = loc.act ? p[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  fft_line<T, L, E, DIR>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+  if (loc.act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) p[(t + m * (L / E)) * rs] = v[m];
+  }
+}
+
+// ---------------------------------------------------------------------------
+// fused axis-0 pass: FFT(N) along axis 0, semi-implicit update, inverse FFT
+// ---------------------------------------------------------------------------
+template <typename T, int L, int TC, bool SAMPLE>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
+void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
+             const cpx<T>* __restrict__ tw, KGeom g, Phys ph, double* __restrict__ partial) {
+  using C = cpx<T>;
+  constexpr int E = elems_per_thread<L>();
+  constexpr int NT = tile_threads<L, TC>();
+  __shared__ C lds[L * (TC + PADT)];
+  const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
+  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
+  const int rs = loc.sg->rs;
+  const bool act = loc.act;
+  C* pw = work + loc.off;
+  C* pp = psi + loc.off;
+
+  // issue every load of the tile up front: N (transformed below) and psi_k
+  C v[E], old[E];
+#pragma unroll
+  for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+#pragma unroll
+  for (int m = 0; m < E; ++m) old[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+
+  // wavenumber of this column
+  int kb, kz;
+  if (loc.sg->kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
+  else { kb = loc.col; kz = g.H; }
+  const double ky = g.dk1 * (double)signed_index(kb, g.n1);
+  const double kzv = g.dk2 * (double)kz;
+  const double kyz2 = ky * ky + kzv * kzv;
+  const double wgt = act ? ((loc.sg->kind == 0 && kz == 0) || loc.sg->kind == 1 ? 1.0 : 2.0) : 0.0;
+  double acc[XPART] = {0.0, 0.0};
+
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int ka = t + m * (L / E);
+    const double kx = g.dk0 * (double)signed_index(ka, g.n0);
+    const double k2 = kx * kx + kyz2;
+    const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    // euler_step.cu:54 denominator and the update, FP64
+    const double den = 1.0 + k2 * ph.dtM * ph.one_m_eps - k2 * ph.dtM * c2;
+    const double inv = 1.0 / den;
+    const double coef = k2 * ph.dtM * inv;
+    const C nh = v[m];
+    const double nx = inv * (double)old[m].x - coef * (double)nh.x;
+    const double ny = inv * (double)old[m].y - coef * (double)nh.y;
+    if constexpr (SAMPLE) {
+      acc[0] += wgt * c2 * (nx * nx + ny * ny);
+      // sse_calculation.cu:133-140 (eps absent from the residual, as there)
+      const double rx = (nx - (double)old[m].x) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old[m].x + (double)nh.x - c2 * (double)old[m].x);
+      const double ry = (ny - (double)old[m].y) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old[m].y + (double)nh.y - c2 * (double)old[m].y);
+      acc[1] += wgt * (rx * rx + ry * ry);
+    }
+    v[m] = mk<C>((T)nx, (T)ny);
+  }
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) pp[(t + m * (L / E)) * rs] = v[m];
+  }
+
+  fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) pw[(t + m * (L / E)) * rs] = v[m];
+  }
+  if constexpr (SAMPLE) block_sum<NT, XPART>(acc, partial + (long long)blockIdx.x * XPART);
+}
+
+// ---------------------------------------------------------------------------
+// real-space pass along the contiguous axis (H-point complex FFT of the
+// even/odd packed line, with the half-spectrum pre/post twiddle)
+// ---------------------------------------------------------------------------
+template <int H> constexpr int z_elems() { return H < 8 ? H : 8; }
+template <int H> constexpr int z_tpl() { return H / z_elems<H>(); }  // threads per line
+template <int H, int NTZ> constexpr bool z_ok() { return NTZ >= z_tpl<H>() && NTZ % z_tpl<H>() == 0; }
+// LDS line: one slot per row plus a gap every 8 rows (breaks the stride-8
+// bank pattern of the Stockham stage writes), plus the X[H] slot.
+template <int H> constexpr int z_len() { return H + H / 8 + 2; }
+template <int H> struct ZIdx {
+  int g;
+  __device__ __forceinline__ int operator()(int r) const { return g * z_len<H>() + r + (r >> 3); }
+};
+
+template <typename T, int H, int NTZ, int MODE, bool SAMPLE>
+__global__ __launch_bounds__(NTZ)
+void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
+             cpx<T>* __restrict__ out_ny, long long nlines, ZArgs za,
+             const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
+             double* __restrict__ partial) {
+  using C = cpx<T>;
+  constexpr int E = z_elems<H>();
+  constexpr int TPL = z_tpl<H>();
+  constexpr int G = NTZ / TPL;
+  __shared__ C lds[G * z_len<H>()];
+  const int tid = threadIdx.x;
+  const int g = tid / TPL, t = tid - g * TPL;
+  const long long line = (long long)blockIdx.x * G + g;
+  const bool act = line < nlines;
+  const ZIdx<H> idx{g};
+  double acc[ZPART] = {0.0, 0.0, 0.0, 0.0};
+  C v[E];
+
+  if constexpr (MODE == Z_STEP || MODE == Z_C2R) {
+    const C* sp = reinterpret_cast<const C*>(in) + line * H;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * TPL;
+      lds[idx(k)] = act ? sp[k] : mk<C>(0, 0);
+    }
+    if (t == 0) lds[idx(H)] = act ? in_ny[line] : mk<C>(0, 0);
+    __syncthreads();
+    const T s = (T)za.scale;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * TPL;
+      C xk = lds[idx(k)];
+      C xm = lds[idx(H - k)];
+      if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist bins are real (C2R)
+      const C e = mk<C>(xk.x + xm.x, xk.y - xm.y);
+      const C d = mk<C>(xk.x - xm.x, xk.y + xm.y);
+      const C o = cmul(d, cconj(twp[k]));
+      v[m] = mk<C>((e.x - o.y) * s, (e.y + o.x) * s);
+    }
+    __syncthreads();
+    fft_line<T, H, E, +1>(v, t, lds, idx, tw);
+    if constexpr (MODE == Z_C2R) {
+      C* rp = reinterpret_cast<C*>(out) + line * H;  // real pairs
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * TPL;
+        const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
+        if constexpr (SAMPLE) {
+          if (act) {
+            acc[0] += p0 + p1;
+            acc[1] += p0 * p0 + p1 * p1;
+            acc[2] += p0 * p0 * p0 + p1 * p1 * p1;
+            acc[3] += (p0 * p0) * (p0 * p0) + (p1 * p1) * (p1 * p1);
+          }
+        }
+        if (act) rp[k] = mk<C>((T)p0, (T)p1);
+      }
+      if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)blockIdx.x * ZPART);
+      return;
+    } else {
+      const T a = (T)za.a, b = (T)za.b;
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
+        if constexpr (SAMPLE) {
+          if (act) {
+            acc[0] += p0 + p1;
+            acc[1] += p0 * p0 + p1 * p1;
+            acc[2] += p0 * p0 * p0 + p1 * p1 * p1;
+            acc[3] += (p0 * p0) * (p0 * p0) + (p1 * p1) * (p1 * p1);
+          }
+        }
+        const T q0 = (T)p0, q1 = (T)p1;
+        v[m] = mk<C>(q0 * q0 * (b * q0 - a), q1 * q1 * (b * q1 - a));
+      }
+    }
+  } else {
+    // input: the fluctuation psi - mean as real pairs (x[2k], x[2k+1])
+    const C* rp = reinterpret_cast<const C*>(in) + line * H;
+    const T a = (T)za.a, b = (T)za.b;
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int k = t + m * TPL;
+      C x = act ? rp[k] : mk<C>(0, 0);
+      if constexpr (MODE == Z_R2C_NL) {
+        const T q0 = (T)(za.mean + (double)x.x), q1 = (T)(za.mean + (double)x.y);
+        x = mk<C>(q0 * q0 * (b * q0 - a), q1 * q1 * (b * q1 - a));
+      }
+      v[m] = act ? x : mk<C>(0, 0);
+    }
+  }
+
+  // forward H-point transform and the real-to-half-spectrum post twiddle
+  fft_line<T, H, E, -1>(v, t, lds, idx, tw);
+  __syncthreads();
+#pragma unroll
+  for (int m = 0; m < E; ++m) lds[idx(t + m * TPL)] = v[m];
+  __syncthreads();
+  C* op = reinterpret_cast<C*>(out) + line * H;
+  const T half = (T)0.5;
+#pragma unroll
+  for (int m = 0; m < E; ++m) {
+    const int k = t + m * TPL;
+    const C zk = lds[idx(k)];
+    const C zm = lds[idx((H - k) & (H - 1))];
+    const C e = mk<C>(half * (zk.x + zm.x), half * (zk.y - zm.y));
+    const C o = mk<C>(half * (zk.y + zm.y), -half * (zk.x - zm.x));
+    if (act) {
+      if (k == 0) {
+        op[0] = mk<C>(e.x + o.x, 0);
+        out_ny[line] = mk<C>(e.x - o.x, 0);
+      } else {
+        op[k] = cadd(e, cmul(twp[k], o));
+      }
+    }
+  }
+  if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)blockIdx.x * ZPART);
+}
+
+// ---------------------------------------------------------------------------
+// final fixed-order reduction of the per-workgroup partials
+// out = {sum c2|psi_k|^2, sum |R_k|^2, sum psi, sum psi^2, sum psi^3, sum psi^4}
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ px, int nx,
+                                                   const double* __restrict__ pz, int nz,
+                                                   double* __restrict__ out) {
+  double a[NSUMS] = {0, 0, 0, 0, 0, 0};
+  for (int i = threadIdx.x; i < nx; i += 1024) {
+    a[0] += px[(long long)i * XPART + 0];
+    a[1] += px[(long long)i * XPART + 1];
+  }
+  for (int i = threadIdx.x; i < nz; i += 1024) {
+#pragma unroll
+    for (int k = 0; k < ZPART; ++k) a[2 + k] += pz[(long long)i * ZPART + k];
+  }
+  block_sum<1024, NSUMS>(a, out);
+}
+
+// k-space Parseval sum  sum_k w c2(k) |psi_k|^2  over the packed spectrum
+template <typename T>
+__global__ __launch_bounds__(256) void k_kspace_sums(const cpx<T>* __restrict__ psi, long long nmain,
+                                                     long long nny, KGeom g, Phys ph,
+                                                     double* __restrict__ partial) {
+  double acc[XPART] = {0.0, 0.0};
+  const long long tot = nmain + nny;
+  for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < tot;
+       e += (long long)gridDim.x * 256) {
+    int ka, kb, kz;
+    double w;
+    if (e < nmain) {
+      kz = (int)(e % g.H);
+      const long long r = e / g.H;
+      kb = (int)(r % g.n1);
+      ka = (int)(r / g.n1);
+      w = kz == 0 ? 1.0 : 2.0;
+    } else {
+      const long long r = e - nmain;
+      kb = (int)(r % g.n1);
+      ka = (int)(r / g.n1);
+      kz = g.H;
+      w = 1.0;
+    }
+    const double kx = g.dk0 * signed_index(ka, g.n0), ky = g.dk1 * signed_index(kb, g.n1),
+                 kzv = g.dk2 * kz;
+    const double k2 = kx * kx + ky * ky + kzv * kzv;
+    const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    const cpx<T> v = psi[e];
+    acc[0] += w * c2 * ((double)v.x * v.x + (double)v.y * v.y);
+  }
+  block_sum<256, XPART>(acc, partial + (long long)blockIdx.x * XPART);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+#define XPFC_LSWITCH(L, ...)                                    \
+  switch (L) {                                                  \
+    case 4: { constexpr int LL = 4; __VA_ARGS__; } break;       \
+    case 8: { constexpr int LL = 8; __VA_ARGS__; } break;       \
+    case 16: { constexpr int LL = 16; __VA_ARGS__; } break;     \
+    case 32: { constexpr int LL = 32; __VA_ARGS__; } break;     \
+    case 64: { constexpr int LL = 64; __VA_ARGS__; } break;     \
+    case 128: { constexpr int LL = 128; __VA_ARGS__; } break;   \
+    case 256: { constexpr int LL = 256; __VA_ARGS__; } break;   \
+    case 512: { constexpr int LL = 512; __VA_ARGS__; } break;   \
+    case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break; \
+    default: return -1;                                         \
+  }
+#define XPFC_TCSWITCH(TCV, ...)                                 \
+  switch (TCV) {                                                \
+    case 4: { constexpr int TT = 4; __VA_ARGS__; } break;       \
+    case 8: { constexpr int TT = 8; __VA_ARGS__; } break;       \
+    case 16: { constexpr int TT = 16; __VA_ARGS__; } break;     \
+    default: return -3;                                         \
+  }
+#define XPFC_NTZSWITCH(V, ...)                                  \
+  switch (V) {                                                  \
+    case 64: { constexpr int NZ = 64; __VA_ARGS__; } break;     \
+    case 128: { constexpr int NZ = 128; __VA_ARGS__; } break;   \
+    case 256: { constexpr int NZ = 256; __VA_ARGS__; } break;   \
+    case 512: { constexpr int NZ = 512; __VA_ARGS__; } break;   \
+    default: return -3;                                         \
+  }
+
+static Segs fill_tiles(const Segs& in, int TC) {
+  Segs s = in;
+  int tiles[2];
+  for (int i = 0; i < 2; ++i) {
+    s.s[i].ntc = (s.s[i].ncols + TC - 1) / TC;
+    tiles[i] = (s.s[i].ncols <= 0 || s.s[i].nbatch <= 0) ? 0 : s.s[i].ntc * s.s[i].nbatch;
+  }
+  s.tiles0 = tiles[0];
+  s.tiles1 = tiles[1];
+  return s;
+}
+
+template <typename T>
+static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const void* tw,
+                     hipStream_t s) {
+  XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
+    if constexpr (tile_ok<T, LL, TT>()) {
+      const Segs sg = fill_tiles(segs0, TT);
+      const int nb = sg.tiles0 + sg.tiles1;
+      if (nb == 0) return 0;
+      if (dir < 0)
+        hipLaunchKernelGGL((k_strided<T, LL, TT, -1>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
+                           (cpx<T>*)buf, sg, (const cpx<T>*)tw);
+      else
+        hipLaunchKernelGGL((k_strided<T, LL, TT, +1>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
+                           (cpx<T>*)buf, sg, (const cpx<T>*)tw);
+    } else {
+      return -3;
+    }
+  }));
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
+                   hipStream_t s) {
+  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, s)
+                   : strided_T<double>(buf, segs, L, tc, dir, tw, s);
+}
+
+template <typename T>
+static int blocks_T(int L, int tc, const Segs& segs0) {
+  XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
+    if constexpr (tile_ok<T, LL, TT>()) {
+      const Segs sg = fill_tiles(segs0, TT);
+      return sg.tiles0 + sg.tiles1;
+    } else {
+      return -3;
+    }
+  }));
+  return -1;
+}
+
+int strided_blocks(int prec, int L, int tc, const Segs& segs) {
+  return prec == 1 ? blocks_T<float>(L, tc, segs) : blocks_T<double>(L, tc, segs);
+}
+
+template <typename T>
+static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, const void* tw,
+                   const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s) {
+  int nb = 0;
+  XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
+    if constexpr (tile_ok<T, LL, TT>()) {
+      const Segs sg = fill_tiles(segs0, TT);
+      nb = sg.tiles0 + sg.tiles1;
+      if (sample)
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, true>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
+                           (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+      else
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, false>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
+                           (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+    } else {
+      return -3;
+    }
+  }));
+  return hipGetLastError() == hipSuccess ? nb : -2;
+}
+
+int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
+                 const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s) {
+  return prec == 1 ? xstep_T<float>(work, psi, segs, L, tc, tw, g, ph, partial, sample, s)
+                   : xstep_T<double>(work, psi, segs, L, tc, tw, g, ph, partial, sample, s);
+}
+
+int zpass_blocks(int H, int ntz, long long nlines) {
+  int G = 0;
+  XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
+    if constexpr (z_ok<LL, NZ>()) G = NZ / z_tpl<LL>();
+    else return -3;
+  }));
+  return (int)((nlines + G - 1) / G);
+}
+
+template <typename T, int H, int NZ, int MODE>
+static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, long long nlines,
+                    const ZArgs& za, const void* tw, const void* twp, double* partial, bool sample,
+                    hipStream_t s) {
+  constexpr int G = NZ / z_tpl<H>();
+  const int nb = (int)((nlines + G - 1) / G);
+  if (sample)
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, MODE, true>), dim3(nb), dim3(NZ), 0, s, in,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
+                       (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
+  else
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, MODE, false>), dim3(nb), dim3(NZ), 0, s, in,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
+                       (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
+}
+
+template <typename T>
+static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
+                   long long nlines, int H, int ntz, const ZArgs& za, const void* tw,
+                   const void* twp, double* partial, bool sample, hipStream_t s) {
+  if (mode == Z_STEP) {
+    XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
+      if constexpr (z_ok<LL, NZ>())
+        zlaunch<T, LL, NZ, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
+      else
+        return -3;
+    }));
+  } else {
+    // set-up and read-back modes: one workgroup shape (256 threads)
+    XPFC_LSWITCH(H, {
+      constexpr int NZ = 256;
+      switch (mode) {
+        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
+        case Z_R2C_NL: zlaunch<T, LL, NZ, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
+        case Z_C2R: zlaunch<T, LL, NZ, Z_C2R>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s); break;
+        default: return -1;
+      }
+    });
+  }
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
+                 long long nlines, int H, int ntz, const ZArgs& za, const void* tw, const void* twp,
+                 double* partial, bool sample, hipStream_t s) {
+  return prec == 1
+             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, ntz, za, tw, twp, partial, sample, s)
+             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, ntz, za, tw, twp, partial, sample, s);
+}
+
+int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, px, nx, pz, nz, out);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
+                       const Phys& ph, double* partial, int* nblocks, hipStream_t s) {
+  long long nb = (nmain + nny + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  if (nb < 1) nb = 1;
+  *nblocks = (int)nb;
+  if (prec == 1)
+    hipLaunchKernelGGL(k_kspace_sums<float>, dim3(nb), dim3(256), 0, s, (const cpx<float>*)psi,
+                       nmain, nny, g, ph, partial);
+  else
+    hipLaunchKernelGGL(k_kspace_sums<double>, dim3(nb), dim3(256), 0, s, (const cpx<double>*)psi,
+                       nmain, nny, g, ph, partial);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+}  // namespace xpfc
