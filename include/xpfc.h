@@ -140,7 +140,9 @@ int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
 int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
 /* Pass variants: v[0] = tile width (columns per workgroup, 4/8/16) of the
  * axis-0 pass, v[1] = of the axis-1 passes, v[2] = workgroup size (64..512)
- * of the real-space pass.  Variants regroup lines into workgroups and never
+ * of the real-space pass, plus 1 to stage the C2R mirror element through LDS
+ * instead of re-reading it from global memory.  Variants regroup lines into
+ * workgroups or change where an operand is read from, and never
  * change the computed field.  xpfc_autotune times every variant on scratch
  * buffers, keeps the fastest (cached per precision and grid for the process)
  * and returns the choice in v (may be NULL); xpfc_step runs it on first use

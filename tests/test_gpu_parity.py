@@ -176,7 +176,7 @@ def test_variants_give_bitwise_identical_fields(shape, prec):
     op, lp = growth_case(shape, prec=prec, t_sample=0)
     psi0 = O.noise_ic(op, 0.01, 9)
     outs = []
-    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512)]:
+    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (0, 0, 257)]:
         with Context(lp) as ctx:
             try:
                 ctx.set_variants(*v)

@@ -417,12 +417,12 @@ extern "C" int xpfc_create(const xpfc_params* pin, xpfc_ctx** out) {
   c->tc_x = default_tc(c->prec, c->n0, sx);
   c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, sy) : 0;
   if (c->tc_x <= 0 || (c->n1 > 1 && c->tc_y <= 0)) return bail(fail(XPFC_ERR_ARG, "unsupported grid"));
-  c->ntz = zpass_blocks(c->H, 256, 1) > 0 ? 256 : 512;
+  c->ntz = zpass_blocks(c->prec, c->H, 256, 1) > 0 ? 256 : 512;
   // partial buffers sized for the largest variant (narrowest tile, smallest block)
   c->npx_cap = 4096;
   for (int tc : {4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, sx));
   c->npz_cap = 1;
-  for (int nz : {64, 128, 256, 512}) c->npz_cap = std::max(c->npz_cap, zpass_blocks(c->H, nz, (long long)c->n0 * c->n1));
+  for (int nz : {64, 128, 256, 512}) c->npz_cap = std::max(c->npz_cap, zpass_blocks(c->prec, c->H, nz, (long long)c->n0 * c->n1));
   if (hipMalloc((void**)&c->px, (size_t)c->npx_cap * XPART * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->pz, (size_t)c->npz_cap * ZPART * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->sums, (size_t)SUM_SLOTS * NSUMS * sizeof(double)) != hipSuccess)
@@ -695,7 +695,7 @@ int launch_one_step(xpfc_ctx* c, bool sample, double* sums_out, hipEvent_t* ev =
   if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   if (sample)
-    LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->H, c->ntz, nl), sums_out, c->stream));
+    LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->prec, c->H, c->ntz, nl), sums_out, c->stream));
   return XPFC_OK;
 }
 
@@ -829,7 +829,7 @@ int autotune(xpfc_ctx* c) {
   HIPCHK(hipMemsetAsync(b, 0, sbytes, c->stream));
   const int reps = c->N >= (1LL << 24) ? 4 : 16;
   int best[3] = {c->tc_x, c->tc_y, c->ntz};
-  const std::vector<int> tcs{4, 8, 16}, ntzs{64, 128, 256, 512};
+  const std::vector<int> tcs{4, 8, 16}, ntzs{64, 128, 256, 512, 65, 129, 257, 513};
   for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
     if (which == 1 && c->n1 <= 1) continue;
     float bt = 1e30f;
@@ -879,7 +879,7 @@ extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
     return fail(XPFC_ERR_ARG, "tile width %d not available for n0 = %d", v[0], c->n0);
   if (c->n1 > 1 && v[1] > 0 && strided_blocks(c->prec, c->n1, v[1], c->segY) < 0)
     return fail(XPFC_ERR_ARG, "tile width %d not available for n1 = %d", v[1], c->n1);
-  if (v[2] > 0 && zpass_blocks(c->H, v[2], nl) < 0)
+  if (v[2] > 0 && zpass_blocks(c->prec, c->H, v[2], nl) < 0)
     return fail(XPFC_ERR_ARG, "real-space block %d not available for n2 = %d", v[2], c->n2);
   if (v[0] > 0) c->tc_x = v[0];
   if (c->n1 > 1 && v[1] > 0) c->tc_y = v[1];
@@ -1011,7 +1011,7 @@ int fe_now(xpfc_ctx* c, double* F, double* fm, double* mean_rs) {
   int rc = psi_to_real(c, true);
   if (rc) return rc;
   double* slot = c->sums + (size_t)(SUM_SLOTS - 1) * NSUMS;
-  LCHK(launch_finalize(c->px, nb, c->pz, zpass_blocks(c->H, 256, (long long)c->n0 * c->n1), slot,
+  LCHK(launch_finalize(c->px, nb, c->pz, zpass_blocks(c->prec, c->H, 256, (long long)c->n0 * c->n1), slot,
                        c->stream));
   double h[NSUMS];
   HIPCHK(hipMemcpyAsync(h, slot, sizeof(h), hipMemcpyDeviceToHost, c->stream));

@@ -49,6 +49,9 @@ struct ZArgs {
 };
 
 enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
+// real-space variant flag or-ed into the workgroup size: stage the C2R mirror
+// element through LDS instead of re-reading it from global memory
+constexpr int ZMIRROR_LDS = 1;
 
 // Per-workgroup FP64 partial sums.  X pass: {sum w c2 |psi_k|^2, sum w |R_k|^2}.
 // Z pass: {sum psi, sum psi^2, sum psi^3, sum psi^4}.
@@ -68,7 +71,7 @@ int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int t
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                  long long nlines, int H, int ntz, const ZArgs& za, const void* tw, const void* twp,
                  double* partial, bool sample, hipStream_t s);
-int zpass_blocks(int H, int ntz, long long nlines);
+int zpass_blocks(int prec, int H, int ntz, long long nlines);
 int strided_blocks(int prec, int L, int tc, const Segs& segs);
 int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s);
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,

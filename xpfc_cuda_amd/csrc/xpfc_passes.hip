@@ -106,12 +106,9 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
-  // issue every load of the tile up front: N (transformed below) and psi_k
-  C v[E], old[E];
+  C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) old[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
 
   // wavenumber of this column
@@ -124,9 +121,14 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   const double wgt = act ? ((loc.sg->kind == 0 && kz == 0) || loc.sg->kind == 1 ? 1.0 : 2.0) : 0.0;
   double acc[XPART] = {0.0, 0.0};
 
-#pragma unroll
+  // One element per iteration, the register array rotated by one each time:
+  // a single copy of the FP64 code keeps the register footprint small.
+  C nxt = act ? pp[t * rs] : mk<C>(0, 0);
+#pragma unroll 1
   for (int m = 0; m < E; ++m) {
     const int ka = t + m * (L / E);
+    const C old = nxt;
+    if (m + 1 < E) nxt = act ? pp[(ka + L / E) * rs] : mk<C>(0, 0);  // prefetch
     const double kx = g.dk0 * (double)signed_index(ka, g.n0);
     const double k2 = kx * kx + kyz2;
     const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
@@ -134,23 +136,23 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
     const double den = 1.0 + k2 * ph.dtM * ph.one_m_eps - k2 * ph.dtM * c2;
     const double inv = 1.0 / den;
     const double coef = k2 * ph.dtM * inv;
-    const C nh = v[m];
-    const double nx = inv * (double)old[m].x - coef * (double)nh.x;
-    const double ny = inv * (double)old[m].y - coef * (double)nh.y;
+    const C nh = v[0];
+    const double nx = inv * (double)old.x - coef * (double)nh.x;
+    const double ny = inv * (double)old.y - coef * (double)nh.y;
     if constexpr (SAMPLE) {
       acc[0] += wgt * c2 * (nx * nx + ny * ny);
       // sse_calculation.cu:133-140 (eps absent from the residual, as there)
-      const double rx = (nx - (double)old[m].x) * ph.inv_dt
-                        + ph.Mres * k2 * ((double)old[m].x + (double)nh.x - c2 * (double)old[m].x);
-      const double ry = (ny - (double)old[m].y) * ph.inv_dt
-                        + ph.Mres * k2 * ((double)old[m].y + (double)nh.y - c2 * (double)old[m].y);
+      const double rx = (nx - (double)old.x) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old.x + (double)nh.x - c2 * (double)old.x);
+      const double ry = (ny - (double)old.y) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old.y + (double)nh.y - c2 * (double)old.y);
       acc[1] += wgt * (rx * rx + ry * ry);
     }
-    v[m] = mk<C>((T)nx, (T)ny);
-  }
-  if (act) {
+    const C nv = mk<C>((T)nx, (T)ny);
+    if (act) pp[ka * rs] = nv;
 #pragma unroll
-    for (int m = 0; m < E; ++m) pp[(t + m * (L / E)) * rs] = v[m];
+    for (int i = 0; i + 1 < E; ++i) v[i] = v[i + 1];
+    v[E - 1] = nv;
   }
 
   fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
@@ -165,26 +167,35 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
 // real-space pass along the contiguous axis (H-point complex FFT of the
 // even/odd packed line, with the half-spectrum pre/post twiddle)
 // ---------------------------------------------------------------------------
-template <int H> constexpr int z_elems() { return H < 8 ? H : 8; }
-template <int H> constexpr int z_tpl() { return H / z_elems<H>(); }  // threads per line
-template <int H, int NTZ> constexpr bool z_ok() { return NTZ >= z_tpl<H>() && NTZ % z_tpl<H>() == 0; }
+// radix-16 FP32 lines from H = 128 up: one LDS exchange per transform at H = 128, 256
+template <typename T, int H> constexpr int z_elems() {
+  return H < 8 ? H : ((H >= 128 && sizeof(T) == 4) ? 16 : 8);
+}
+template <typename T, int H> constexpr int z_tpl() { return H / z_elems<T, H>(); }  // threads per line
+template <typename T, int H, int NTZ> constexpr bool z_ok() {
+  return NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0;
+}
 // LDS line: one slot per row plus a gap every 8 rows (breaks the stride-8
 // bank pattern of the Stockham stage writes), plus the X[H] slot.
-template <int H> constexpr int z_len() { return H + H / 8 + 2; }
+template <int H> constexpr int z_len() { return H + H / 8 + 3; }
 template <int H> struct ZIdx {
   int g;
   __device__ __forceinline__ int operator()(int r) const { return g * z_len<H>() + r + (r >> 3); }
 };
 
-template <typename T, int H, int NTZ, int MODE, bool SAMPLE>
-__global__ __launch_bounds__(NTZ)
+// MIRROR_LDS: the C2R pre-twiddle needs X[k] and X[H-k]; with MIRROR_LDS the
+// line is staged through LDS, otherwise the mirror element is re-read from
+// global memory (a cache hit: this workgroup is loading that line).  Both give
+// the same values, so this is a tuning variant like the workgroup size.
+template <typename T, int H, int NTZ, bool MIRROR_LDS, int MODE, bool SAMPLE>
+__global__ __launch_bounds__(NTZ, (sizeof(T) == 4 ? 4 : 2))
 void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
              cpx<T>* __restrict__ out_ny, long long nlines, ZArgs za,
              const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
              double* __restrict__ partial) {
   using C = cpx<T>;
-  constexpr int E = z_elems<H>();
-  constexpr int TPL = z_tpl<H>();
+  constexpr int E = z_elems<T, H>();
+  constexpr int TPL = z_tpl<T, H>();
   constexpr int G = NTZ / TPL;
   __shared__ C lds[G * z_len<H>()];
   const int tid = threadIdx.x;
@@ -197,26 +208,34 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
 
   if constexpr (MODE == Z_STEP || MODE == Z_C2R) {
     const C* sp = reinterpret_cast<const C*>(in) + line * H;
-#pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = t + m * TPL;
-      lds[idx(k)] = act ? sp[k] : mk<C>(0, 0);
-    }
-    if (t == 0) lds[idx(H)] = act ? in_ny[line] : mk<C>(0, 0);
-    __syncthreads();
     const T s = (T)za.scale;
+    if constexpr (MIRROR_LDS) {
+#pragma unroll
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * TPL;
+        lds[idx(k)] = act ? sp[k] : mk<C>(0, 0);
+      }
+      if (t == 0) lds[idx(H)] = act ? in_ny[line] : mk<C>(0, 0);
+      __syncthreads();
+    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = t + m * TPL;
-      C xk = lds[idx(k)];
-      C xm = lds[idx(H - k)];
+      C xk, xm;
+      if constexpr (MIRROR_LDS) {
+        xk = lds[idx(k)];
+        xm = lds[idx(H - k)];
+      } else {
+        xk = act ? sp[k] : mk<C>(0, 0);
+        xm = act ? (k == 0 ? in_ny[line] : sp[H - k]) : mk<C>(0, 0);
+      }
       if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist bins are real (C2R)
       const C e = mk<C>(xk.x + xm.x, xk.y - xm.y);
       const C d = mk<C>(xk.x - xm.x, xk.y + xm.y);
       const C o = cmul(d, cconj(twp[k]));
       v[m] = mk<C>((e.x - o.y) * s, (e.y + o.x) * s);
     }
-    __syncthreads();
+    if constexpr (MIRROR_LDS) __syncthreads();  // the transform reuses the LDS line
     fft_line<T, H, E, +1>(v, t, lds, idx, tw);
     if constexpr (MODE == Z_C2R) {
       C* rp = reinterpret_cast<C*>(out) + line * H;  // real pairs
@@ -464,50 +483,63 @@ int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int t
                    : xstep_T<double>(work, psi, segs, L, tc, tw, g, ph, partial, sample, s);
 }
 
-int zpass_blocks(int H, int ntz, long long nlines) {
+template <typename T>
+static int zblocks_T(int H, int ntz, long long nlines) {
+  ntz &= ~ZMIRROR_LDS;
   int G = 0;
   XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
-    if constexpr (z_ok<LL, NZ>()) G = NZ / z_tpl<LL>();
+    if constexpr (z_ok<T, LL, NZ>()) G = NZ / z_tpl<T, LL>();
     else return -3;
   }));
   return (int)((nlines + G - 1) / G);
 }
 
-template <typename T, int H, int NZ, int MODE>
+int zpass_blocks(int prec, int H, int ntz, long long nlines) {
+  return prec == 1 ? zblocks_T<float>(H, ntz, nlines) : zblocks_T<double>(H, ntz, nlines);
+}
+
+template <typename T, int H, int NZ, bool MIR, int MODE>
 static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, long long nlines,
                     const ZArgs& za, const void* tw, const void* twp, double* partial, bool sample,
                     hipStream_t s) {
-  constexpr int G = NZ / z_tpl<H>();
+  constexpr int G = NZ / z_tpl<T, H>();
   const int nb = (int)((nlines + G - 1) / G);
   if (sample)
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, MODE, true>), dim3(nb), dim3(NZ), 0, s, in,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, MIR, MODE, true>), dim3(nb), dim3(NZ), 0, s, in,
                        (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
   else
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, MODE, false>), dim3(nb), dim3(NZ), 0, s, in,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, MIR, MODE, false>), dim3(nb), dim3(NZ), 0, s, in,
                        (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
 }
 
+// ntz: workgroup size (64..512); +ZMIRROR_LDS stages the C2R mirror through LDS
 template <typename T>
 static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                    long long nlines, int H, int ntz, const ZArgs& za, const void* tw,
                    const void* twp, double* partial, bool sample, hipStream_t s) {
+  const bool mir = (ntz & ZMIRROR_LDS) != 0;
+  ntz &= ~ZMIRROR_LDS;
   if (mode == Z_STEP) {
     XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
-      if constexpr (z_ok<LL, NZ>())
-        zlaunch<T, LL, NZ, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
-      else
+      if constexpr (z_ok<T, LL, NZ>()) {
+        if (mir)
+          zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
+        else
+          zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
+      } else {
         return -3;
+      }
     }));
   } else {
     // set-up and read-back modes: one workgroup shape (256 threads)
     XPFC_LSWITCH(H, {
       constexpr int NZ = 256;
       switch (mode) {
-        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
-        case Z_R2C_NL: zlaunch<T, LL, NZ, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
-        case Z_C2R: zlaunch<T, LL, NZ, Z_C2R>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s); break;
+        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
+        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
+        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s); break;
         default: return -1;
       }
     });
