@@ -7,13 +7,14 @@ namespace xpfc {
 
 // DCF (initialize.cu:209-217): sum or max over peaks of
 // amp_i * exp(-(|k| - k_i)^2 / (2 alpha_i^2)), amp_i = exp(-(sigma k_i)^2/(2 rho_i B_i)).
-// Terms with exponent >= 50 (below 2e-22 relative) are skipped.
+// Terms with exponent >= ph.ecut (negligible in the denominator's precision)
+// are skipped.
 __device__ __forceinline__ double c2hat(double kmag, const Phys& ph) {
   double r = 0.0;
   for (int i = 0; i < ph.npk; ++i) {
     const double d = kmag - ph.pk[i];
     const double e = d * d * ph.inv2a2[i];
-    const double term = (e < 50.0) ? ph.amp[i] * exp(-e) : 0.0;
+    const double term = (e < ph.ecut) ? ph.amp[i] * exp(-e) : 0.0;
     r = ph.dcf_max ? fmax(r, term) : r + term;
   }
   return r;

@@ -442,12 +442,15 @@ extern "C" int xpfc_create(const xpfc_params* pin, xpfc_ctx** out) {
   ph.Mres = p.int_Mo_residual ? (double)(int)p.Mo : p.Mo;
   ph.npk = p.npeaks;
   ph.dcf_max = p.dcf_mode == XPFC_DCF_MAX;
+  // a peak term below exp(-ecut) is dropped: 1e-10 (FP32 fields) / 4e-18 (FP64)
+  // of the denominator, which is >= 1
+  ph.ecut = c->prec == 1 ? 23.0 : 40.0;
   double kc = 0.0;
   for (int i = 0; i < p.npeaks; ++i) {
     ph.pk[i] = p.peak_k[i];
     ph.inv2a2[i] = 1.0 / (2.0 * p.peak_alpha[i] * p.peak_alpha[i]);
     ph.amp[i] = std::exp(-(p.sigma * p.peak_k[i]) * (p.sigma * p.peak_k[i]) / (2.0 * p.peak_rho_beta[i]));
-    kc = std::max(kc, p.peak_k[i] + std::sqrt(50.0 / ph.inv2a2[i]));
+    kc = std::max(kc, p.peak_k[i] + std::sqrt(ph.ecut / ph.inv2a2[i]));
   }
   ph.kcut2 = kc * kc;
   c->c2zero = c2hat_host(0.0, p);
@@ -829,11 +832,15 @@ int autotune(xpfc_ctx* c) {
   HIPCHK(hipMemsetAsync(b, 0, sbytes, c->stream));
   const int reps = c->N >= (1LL << 24) ? 4 : 16;
   int best[3] = {c->tc_x, c->tc_y, c->ntz};
-  const std::vector<int> tcs{4, 8, 16}, ntzs{64, 128, 256, 512, 65, 129, 257, 513};
+  // axis-0: tile width (+XOCC_HINT); axis-1: tile width; real space: block (+ZFREE_REGS)
+  const std::vector<int> xs{4, 8, 16, 4 | XOCC_HINT, 8 | XOCC_HINT, 16 | XOCC_HINT};
+  const std::vector<int> ys{4, 8, 16};
+  const std::vector<int> zs{64, 128, 256, 512, 64 | ZFREE_REGS, 128 | ZFREE_REGS,
+                            256 | ZFREE_REGS, 512 | ZFREE_REGS};
   for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
     if (which == 1 && c->n1 <= 1) continue;
     float bt = 1e30f;
-    for (int v : (which == 2 ? ntzs : tcs)) {
+    for (int v : (which == 0 ? xs : which == 1 ? ys : zs)) {
       float ms = 0.f;
       const int r = time_pass(c, which, v, a, b, reps, &ms);
       if (r == -3) continue;

@@ -35,7 +35,9 @@ struct Phys {
   double one_m_eps;  // 1 - eps
   double inv_dt;     // 1 / dt          (residual)
   double Mres;       // residual mobility (Mo, or (int)Mo like the reference)
-  double kcut2;      // |k|^2 beyond which every DCF peak term is below 2^-72
+  double ecut;       // peak terms exp(-e) with e >= ecut are dropped (below the
+                     // working precision of the denominator: 1e-10 FP32, 4e-18 FP64)
+  double kcut2;      // |k|^2 beyond which every peak term has e >= ecut
   int npk;
   int dcf_max;       // 0 = sum of peaks, 1 = max over peaks
   double pk[4], inv2a2[4], amp[4];
@@ -49,9 +51,11 @@ struct ZArgs {
 };
 
 enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
-// real-space variant flag or-ed into the workgroup size: stage the C2R mirror
-// element through LDS instead of re-reading it from global memory
-constexpr int ZMIRROR_LDS = 1;
+// Register-allocation variant flags (the arithmetic is identical):
+// or-ed into the real-space workgroup size: no occupancy hint;
+// or-ed into the axis-0 tile width: budget for two 1024-thread workgroups/CU.
+constexpr int ZFREE_REGS = 1;
+constexpr int XOCC_HINT = 1;
 
 // Per-workgroup FP64 partial sums.  X pass: {sum w c2 |psi_k|^2, sum w |R_k|^2}.
 // Z pass: {sum psi, sum psi^2, sum psi^3, sum psi^4}.

@@ -91,8 +91,10 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
 // ---------------------------------------------------------------------------
 // fused axis-0 pass: FFT(N) along axis 0, semi-implicit update, inverse FFT
 // ---------------------------------------------------------------------------
-template <typename T, int L, int TC, bool SAMPLE>
-__global__ __launch_bounds__((tile_threads<L, TC>()))
+// OCC: register budget for two 1024-thread workgroups per CU (FP32: 64 VGPRs,
+// FP64: 128), at the price of spills; a register-allocation variant only.
+template <typename T, int L, int TC, bool SAMPLE, bool OCC>
+__global__ __launch_bounds__((tile_threads<L, TC>()), (OCC ? (sizeof(T) == 4 ? 8 : 4) : 1))
 void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
              const cpx<T>* __restrict__ tw, KGeom g, Phys ph, double* __restrict__ partial) {
   using C = cpx<T>;
@@ -121,25 +123,33 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   const double wgt = act ? ((loc.sg->kind == 0 && kz == 0) || loc.sg->kind == 1 ? 1.0 : 2.0) : 0.0;
   double acc[XPART] = {0.0, 0.0};
 
-  // One element per iteration, the register array rotated by one each time:
-  // a single copy of the FP64 code keeps the register footprint small.
+  // psi_k' = (psi_k - dt M k^2 N_k) / den  (euler_step.cu:50-59).  The
+  // denominator and its reciprocal are FP64; the numerator is field arithmetic
+  // in the working precision.  Each element's FP64 chain is fenced off with a
+  // scheduling barrier so the unrolled iterations do not interleave and the
+  // register footprint stays that of one element.
   C nxt = act ? pp[t * rs] : mk<C>(0, 0);
-#pragma unroll 1
+#pragma unroll
   for (int m = 0; m < E; ++m) {
+    __builtin_amdgcn_sched_barrier(0);
     const int ka = t + m * (L / E);
     const C old = nxt;
     if (m + 1 < E) nxt = act ? pp[(ka + L / E) * rs] : mk<C>(0, 0);  // prefetch
     const double kx = g.dk0 * (double)signed_index(ka, g.n0);
     const double k2 = kx * kx + kyz2;
     const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
-    // euler_step.cu:54 denominator and the update, FP64
-    const double den = 1.0 + k2 * ph.dtM * ph.one_m_eps - k2 * ph.dtM * c2;
-    const double inv = 1.0 / den;
-    const double coef = k2 * ph.dtM * inv;
-    const C nh = v[0];
-    const double nx = inv * (double)old.x - coef * (double)nh.x;
-    const double ny = inv * (double)old.y - coef * (double)nh.y;
+    const double a2 = k2 * ph.dtM;
+    const double den = 1.0 + a2 * ph.one_m_eps - a2 * c2;  // euler_step.cu:54
+    double inv = __builtin_amdgcn_rcp(den);                 // v_rcp_f64 + 2 Newton steps
+    double e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    e = fma(-den, inv, 1.0);
+    inv = fma(inv, e, inv);
+    const C nh = v[m];
+    const T ti = (T)inv, ta = (T)a2;
+    const C nv = mk<C>(ti * (old.x - ta * nh.x), ti * (old.y - ta * nh.y));
     if constexpr (SAMPLE) {
+      const double nx = (double)nv.x, ny = (double)nv.y;
       acc[0] += wgt * c2 * (nx * nx + ny * ny);
       // sse_calculation.cu:133-140 (eps absent from the residual, as there)
       const double rx = (nx - (double)old.x) * ph.inv_dt
@@ -148,12 +158,10 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
                         + ph.Mres * k2 * ((double)old.y + (double)nh.y - c2 * (double)old.y);
       acc[1] += wgt * (rx * rx + ry * ry);
     }
-    const C nv = mk<C>((T)nx, (T)ny);
     if (act) pp[ka * rs] = nv;
-#pragma unroll
-    for (int i = 0; i + 1 < E; ++i) v[i] = v[i + 1];
-    v[E - 1] = nv;
+    v[m] = nv;
   }
+  __builtin_amdgcn_sched_barrier(0);
 
   fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
   if (act) {
@@ -175,20 +183,25 @@ template <typename T, int H> constexpr int z_tpl() { return H / z_elems<T, H>();
 template <typename T, int H, int NTZ> constexpr bool z_ok() {
   return NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0;
 }
-// LDS line: one slot per row plus a gap every 8 rows (breaks the stride-8
-// bank pattern of the Stockham stage writes), plus the X[H] slot.
-template <int H> constexpr int z_len() { return H + H / 8 + 3; }
-template <int H> struct ZIdx {
+// LDS line of H rows plus the X[H] slot.  Rows are XOR-swizzled within each
+// block of E: a Stockham stage writes rows E*j + q from lanes j, and the swizzle
+// spreads those lanes over distinct banks; the odd line length puts
+// neighbouring lines on the other half of the banks.
+template <int H> constexpr int z_len() { return H + 1; }
+template <int H, int E> struct ZIdx {
   int g;
-  __device__ __forceinline__ int operator()(int r) const { return g * z_len<H>() + r + (r >> 3); }
+  __device__ __forceinline__ int operator()(int r) const {
+    const int s = (r >= H) ? r : (r ^ ((r / E) & (E - 1)));
+    return g * z_len<H>() + s;
+  }
 };
 
-// MIRROR_LDS: the C2R pre-twiddle needs X[k] and X[H-k]; with MIRROR_LDS the
-// line is staged through LDS, otherwise the mirror element is re-read from
-// global memory (a cache hit: this workgroup is loading that line).  Both give
-// the same values, so this is a tuning variant like the workgroup size.
-template <typename T, int H, int NTZ, bool MIRROR_LDS, int MODE, bool SAMPLE>
-__global__ __launch_bounds__(NTZ, (sizeof(T) == 4 ? 4 : 2))
+// The C2R pre-twiddle needs X[k] and X[H-k]: the mirror element is re-read
+// from global memory (a cache hit: this workgroup is loading that line).
+// FREE: no occupancy hint (the compiler may use more registers instead of
+// spilling); a register-allocation variant only, the arithmetic is identical.
+template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE>
+__global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
 void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
              cpx<T>* __restrict__ out_ny, long long nlines, ZArgs za,
              const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
@@ -202,40 +215,24 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   const int g = tid / TPL, t = tid - g * TPL;
   const long long line = (long long)blockIdx.x * G + g;
   const bool act = line < nlines;
-  const ZIdx<H> idx{g};
+  const ZIdx<H, E> idx{g};
   double acc[ZPART] = {0.0, 0.0, 0.0, 0.0};
   C v[E];
 
   if constexpr (MODE == Z_STEP || MODE == Z_C2R) {
     const C* sp = reinterpret_cast<const C*>(in) + line * H;
     const T s = (T)za.scale;
-    if constexpr (MIRROR_LDS) {
-#pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = t + m * TPL;
-        lds[idx(k)] = act ? sp[k] : mk<C>(0, 0);
-      }
-      if (t == 0) lds[idx(H)] = act ? in_ny[line] : mk<C>(0, 0);
-      __syncthreads();
-    }
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = t + m * TPL;
-      C xk, xm;
-      if constexpr (MIRROR_LDS) {
-        xk = lds[idx(k)];
-        xm = lds[idx(H - k)];
-      } else {
-        xk = act ? sp[k] : mk<C>(0, 0);
-        xm = act ? (k == 0 ? in_ny[line] : sp[H - k]) : mk<C>(0, 0);
-      }
+      C xk = act ? sp[k] : mk<C>(0, 0);
+      C xm = act ? (k == 0 ? in_ny[line] : sp[H - k]) : mk<C>(0, 0);
       if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist bins are real (C2R)
       const C e = mk<C>(xk.x + xm.x, xk.y - xm.y);
       const C d = mk<C>(xk.x - xm.x, xk.y + xm.y);
       const C o = cmul(d, cconj(twp[k]));
       v[m] = mk<C>((e.x - o.y) * s, (e.y + o.x) * s);
     }
-    if constexpr (MIRROR_LDS) __syncthreads();  // the transform reuses the LDS line
     fft_line<T, H, E, +1>(v, t, lds, idx, tw);
     if constexpr (MODE == Z_C2R) {
       C* rp = reinterpret_cast<C*>(out) + line * H;  // real pairs
@@ -441,6 +438,7 @@ int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir
 
 template <typename T>
 static int blocks_T(int L, int tc, const Segs& segs0) {
+  tc &= ~XOCC_HINT;
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
@@ -456,20 +454,27 @@ int strided_blocks(int prec, int L, int tc, const Segs& segs) {
   return prec == 1 ? blocks_T<float>(L, tc, segs) : blocks_T<double>(L, tc, segs);
 }
 
+// tc: tile width, plus XOCC_HINT for the two-workgroups-per-CU register budget
+// (sampled steps always use the unconstrained build)
 template <typename T>
 static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, const void* tw,
                    const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s) {
   int nb = 0;
+  const bool occ = (tc & XOCC_HINT) != 0;
+  tc &= ~XOCC_HINT;
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       nb = sg.tiles0 + sg.tiles1;
       if (sample)
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, true>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
-                           (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, true, false>), dim3(nb), dim3(tile_threads<LL, TT>()),
+                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+      else if (occ)
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, true>), dim3(nb), dim3(tile_threads<LL, TT>()),
+                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
       else
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, false>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
-                           (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, false>), dim3(nb), dim3(tile_threads<LL, TT>()),
+                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
     } else {
       return -3;
     }
@@ -485,7 +490,7 @@ int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int t
 
 template <typename T>
 static int zblocks_T(int H, int ntz, long long nlines) {
-  ntz &= ~ZMIRROR_LDS;
+  ntz &= ~ZFREE_REGS;
   int G = 0;
   XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
     if constexpr (z_ok<T, LL, NZ>()) G = NZ / z_tpl<T, LL>();
@@ -514,17 +519,17 @@ static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, 
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
 }
 
-// ntz: workgroup size (64..512); +ZMIRROR_LDS stages the C2R mirror through LDS
+// ntz: workgroup size (64..512); +ZFREE_REGS drops the occupancy hint
 template <typename T>
 static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                    long long nlines, int H, int ntz, const ZArgs& za, const void* tw,
                    const void* twp, double* partial, bool sample, hipStream_t s) {
-  const bool mir = (ntz & ZMIRROR_LDS) != 0;
-  ntz &= ~ZMIRROR_LDS;
+  const bool fre = (ntz & ZFREE_REGS) != 0;
+  ntz &= ~ZFREE_REGS;
   if (mode == Z_STEP) {
     XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
       if constexpr (z_ok<T, LL, NZ>()) {
-        if (mir)
+        if (fre)
           zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
         else
           zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
