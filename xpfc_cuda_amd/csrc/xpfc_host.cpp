@@ -512,12 +512,13 @@ int upload_fluct(xpfc_ctx* c, const double* host, double mean) {
 // full forward transform of a spectrum-layout buffer whose z axis is already done
 int forward_xy(xpfc_ctx* c, void* spec, bool do_x) {
   if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, c->stream));
-  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, c->stream));
+  if (do_x)
+    LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, -1, c->tw0, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec) {
-  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, c->stream));
+  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, +1, c->tw0, c->stream));
   if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, c->stream));
   return XPFC_OK;
 }
