@@ -108,13 +108,9 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
-  // every load of the tile is issued up front: N (transformed below) and
-  // psi_k (in flight during the forward transform)
-  C v[E], old[E];
+  C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) old[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
 
   // wavenumber of this column
@@ -132,11 +128,13 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   // in the working precision.  Each element's FP64 chain is fenced off with a
   // scheduling barrier so the unrolled iterations do not interleave and the
   // register footprint stays that of one element.
+  C nxt = act ? pp[t * rs] : mk<C>(0, 0);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     __builtin_amdgcn_sched_barrier(0);
     const int ka = t + m * (L / E);
-    const C o = old[m];
+    const C old = nxt;
+    if (m + 1 < E) nxt = act ? pp[(ka + L / E) * rs] : mk<C>(0, 0);  // prefetch
     const double kx = g.dk0 * (double)signed_index(ka, g.n0);
     const double k2 = kx * kx + kyz2;
     const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
@@ -149,15 +147,15 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
     inv = fma(inv, e, inv);
     const C nh = v[m];
     const T ti = (T)inv, ta = (T)a2;
-    const C nv = mk<C>(ti * (o.x - ta * nh.x), ti * (o.y - ta * nh.y));
+    const C nv = mk<C>(ti * (old.x - ta * nh.x), ti * (old.y - ta * nh.y));
     if constexpr (SAMPLE) {
       const double nx = (double)nv.x, ny = (double)nv.y;
       acc[0] += wgt * c2 * (nx * nx + ny * ny);
       // sse_calculation.cu:133-140 (eps absent from the residual, as there)
-      const double rx = (nx - (double)o.x) * ph.inv_dt
-                        + ph.Mres * k2 * ((double)o.x + (double)nh.x - c2 * (double)o.x);
-      const double ry = (ny - (double)o.y) * ph.inv_dt
-                        + ph.Mres * k2 * ((double)o.y + (double)nh.y - c2 * (double)o.y);
+      const double rx = (nx - (double)old.x) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old.x + (double)nh.x - c2 * (double)old.x);
+      const double ry = (ny - (double)old.y) * ph.inv_dt
+                        + ph.Mres * k2 * ((double)old.y + (double)nh.y - c2 * (double)old.y);
       acc[1] += wgt * (rx * rx + ry * ry);
     }
     if (act) pp[ka * rs] = nv;
