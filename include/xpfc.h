@@ -150,6 +150,10 @@ int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
 int xpfc_autotune(xpfc_ctx* ctx, int* v);
 int xpfc_set_variants(xpfc_ctx* ctx, const int* v);
 int xpfc_get_variants(const xpfc_ctx* ctx, int* v);
+/* Step blocks: runs of unsampled steps are captured once as hipGraphs of up to
+ * 50 steps and replayed (on by default; XPFC_GRAPHS=0 or on = 0 launches every
+ * kernel eagerly).  Both give identical fields. */
+int xpfc_set_graphs(xpfc_ctx* ctx, int on);
 /* Block until all work of the context has finished. */
 int xpfc_sync(xpfc_ctx* ctx);
 

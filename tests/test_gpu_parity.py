@@ -196,3 +196,25 @@ def test_autotune_picks_available_variant():
         v = ctx.autotune()
         assert (v[0] & ~1) in (4, 8, 16) and v[1] in (4, 8, 16) and (v[2] & ~1) in (64, 128, 256, 512)
         assert ctx.variants() == v
+
+
+@pytest.mark.parametrize("prec", PRECS)
+def test_graph_replays_match_eager(prec):
+    """Step blocks replayed from captured graphs (several replays, eager work and
+    synchronisation between them) give the same field as eager launches."""
+    op, lp = growth_case((32, 16, 32), prec=prec, t_sample=25)
+    psi0 = O.noise_ic(op, 0.01, 13)
+    fields = []
+    for graphs in (False, True):
+        with Context(lp) as ctx:
+            ctx.set_graphs(graphs)
+            ctx.set_field(psi0)
+            for _ in range(4):        # 4 x 30 steps: replays of the 23- and 24-step blocks
+                ctx.step(30)
+                ctx.fe_calc()         # eager work between replays
+                ctx.sync()
+            fields.append((ctx.get_field(), [(s.step, s.F_ref, s.rmse) for s in ctx.samples()]))
+    assert np.array_equal(fields[0][0], fields[1][0])
+    a = np.array(fields[0][1], dtype=float)
+    b = np.array(fields[1][1], dtype=float)
+    assert a.shape == b.shape and np.array_equal(a, b, equal_nan=True)

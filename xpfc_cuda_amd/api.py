@@ -92,6 +92,7 @@ def load_library(path: Optional[str] = None):
         "xpfc_autotune": (i, [vp, P(i)]),
         "xpfc_set_variants": (i, [vp, P(i)]),
         "xpfc_get_variants": (i, [vp, P(i)]),
+        "xpfc_set_graphs": (i, [vp, i]),
         "xpfc_fe_calc": (i, [vp, P(d), P(d)]),
         "xpfc_avg_calc": (i, [vp, P(d)]),
         "xpfc_sse_calc": (i, [vp, P(d)]),
@@ -379,6 +380,9 @@ class Context:
         v = (ct.c_int * 3)()
         _check(lib().xpfc_get_variants(self.handle, v), "xpfc_get_variants")
         return tuple(v)
+
+    def set_graphs(self, on: bool):
+        _check(lib().xpfc_set_graphs(self.handle, int(bool(on))), "xpfc_set_graphs")
 
     def sync(self):
         _check(lib().xpfc_sync(self.handle), "xpfc_sync")

@@ -93,6 +93,9 @@ struct xpfc_ctx {
   double last_rmse = NAN;
   bool have_rmse = false;
   double c2zero = 0.0;
+  // hipGraph step blocks: runs of unsampled steps, captured once per length
+  bool use_graphs = true;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
 };
 
 // ---------------------------------------------------------------------------
@@ -336,7 +339,15 @@ int default_tc(int prec, int L, const Segs& s) {
   return -1;
 }
 
+// Captured step blocks bake in buffer pointers, pass variants and the mean
+// density (a kernel argument): drop them whenever one of those changes.
+void drop_graphs(xpfc_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
 void free_all(xpfc_ctx* c) {
+  drop_graphs(c);
   void* ptrs[] = {c->psi, c->work, c->aux, c->tw0, c->tw1, c->twz, c->twp,
                   c->px, c->pz, c->sums};
   for (void* q : ptrs)
@@ -459,6 +470,8 @@ extern "C" int xpfc_create(const xpfc_params* pin, xpfc_ctx** out) {
   c->za.scale = 1.0 / (double)c->N;
   c->za.mean = p.psi0;
   c->mean = p.psi0;
+  const char* eg = std::getenv("XPFC_GRAPHS");
+  c->use_graphs = !(eg && eg[0] == '0');
   *out = c;
   return XPFC_OK;
 }
@@ -559,6 +572,7 @@ extern "C" int xpfc_set_field(xpfc_ctx* c, const double* host) {
   if ((rc = forward_xy(c, c->work, false))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->have_field = true;
+  drop_graphs(c);  // the mean density is a captured kernel argument
   return XPFC_OK;
 }
 
@@ -700,6 +714,57 @@ int launch_one_step(xpfc_ctx* c, bool sample, double* sums_out, hipEvent_t* ev =
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   if (sample)
     LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->prec, c->H, c->ntz, nl), sums_out, c->stream));
+  return XPFC_OK;
+}
+
+// Unsampled steps run as replays of captured hipGraphs of up to GRAPH_BLOCK
+// steps (one graph per run length, cached per context).  The captured work is
+// the same launch sequence as launch_one_step, so a replay and an eager run
+// give identical fields.
+constexpr int GRAPH_BLOCK = 50;
+
+int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
+  for (auto& g : c->graphs)
+    if (g.first == len) {
+      *out = g.second;
+      return XPFC_OK;
+    }
+  if (c->graphs.size() >= 8) drop_graphs(c);
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = XPFC_OK;
+  for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = launch_one_step(c, false, nullptr);
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  if (rc != XPFC_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != XPFC_OK ? rc : fail(XPFC_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+  c->graphs.emplace_back(len, ex);
+  *out = ex;
+  return XPFC_OK;
+}
+
+int run_plain_steps(xpfc_ctx* c, long long run) {
+  while (run > 0) {
+    if (!c->use_graphs) {
+      int rc = launch_one_step(c, false, nullptr);
+      if (rc) return rc;
+      c->step++;
+      --run;
+      continue;
+    }
+    const int len = (int)std::min<long long>(run, GRAPH_BLOCK);
+    hipGraphExec_t ex = nullptr;
+    int rc = plain_graph(c, len, &ex);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+    c->step += len;
+    run -= len;
+  }
   return XPFC_OK;
 }
 
@@ -857,6 +922,7 @@ int autotune(xpfc_ctx* c) {
   c->tc_y = best[1];
   c->ntz = best[2];
   c->tuned = true;
+  drop_graphs(c);
   std::lock_guard<std::mutex> lk(g_tune_mu);
   g_tune_cache[key] = {best[0], best[1], best[2]};
   return XPFC_OK;
@@ -893,6 +959,14 @@ extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
   if (c->n1 > 1 && v[1] > 0) c->tc_y = v[1];
   if (v[2] > 0) c->ntz = v[2];
   c->tuned = true;  // explicit choice: the autotuner leaves it alone
+  drop_graphs(c);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_graphs(xpfc_ctx* c, int on) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  c->use_graphs = on != 0;
+  if (!c->use_graphs) drop_graphs(c);
   return XPFC_OK;
 }
 
@@ -926,22 +1000,29 @@ extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
     r.rmse = NAN;
     c->samples.push_back(r);
   }
-  for (int i = 0; i < nsteps; ++i) {
+  long long left = nsteps;
+  while (left > 0) {
     const long long n = c->step;
-    const bool smp = sample_step(c, n);
-    double* slot = nullptr;
-    if (smp) {
+    if (sample_step(c, n)) {
       if ((int)c->pending.size() == SUM_SLOTS) {
         int rc = drain(c);
         if (rc) return rc;
       }
       const int s = (int)c->pending.size();
-      slot = c->sums + (size_t)s * NSUMS;
+      double* slot = c->sums + (size_t)s * NSUMS;
       c->pending.emplace_back(s, n);
+      int rc = launch_one_step(c, true, slot);
+      if (rc) return rc;
+      c->step++;
+      --left;
+      continue;
     }
-    int rc = launch_one_step(c, smp, slot);
+    // a run of unsampled steps up to the next sampled one
+    long long run = 1;
+    while (run < left && !sample_step(c, n + run)) ++run;
+    int rc = run_plain_steps(c, run);
     if (rc) return rc;
-    c->step++;
+    left -= run;
   }
   return drain(c);
 }
