@@ -199,6 +199,35 @@ int xpfc_write_vtk(xpfc_ctx* ctx, const char* path);
 int xpfc_read_vtk(const char* path, int* nx, int* ny, double* spacing, double* host_real,
                   long long capacity);
 
+/* ---- several simulations of one grid in one context ------------------------ */
+/* No counterpart in the reference, which runs one simulation per process
+ * (main.cu:20-435).  Instance s of a context keeps its own physics (every
+ * field of xpfc_params except the grid and precision may differ), field, step
+ * counter and sample log; xpfc_step advances every ACTIVE instance with one
+ * launch per pass.  An instance's field and samples are identical to those of
+ * a one-instance context given the same parameters and field.  The functions
+ * above act on instance 0. */
+int xpfc_create_batch(const xpfc_params* p, int ninst, xpfc_ctx** out);  /* p[ninst] */
+int xpfc_num_instances(const xpfc_ctx* ctx);
+/* Active list (NULL = all instances, in order); stepping skips the others. */
+int xpfc_set_active(xpfc_ctx* ctx, const int* inst, int n);
+/* Replace instance s's physics (its field is kept; set one after this). */
+int xpfc_inst_set_params(xpfc_ctx* ctx, int s, const xpfc_params* p);
+/* Step counter and sample log back to zero (a slot handed to a new job). */
+int xpfc_inst_reset(xpfc_ctx* ctx, int s);
+int xpfc_inst_initialize(xpfc_ctx* ctx, int s);
+int xpfc_inst_initialize_noise(xpfc_ctx* ctx, int s, double amplitude, unsigned long long seed);
+int xpfc_inst_set_field(xpfc_ctx* ctx, int s, const double* host_real);
+int xpfc_inst_get_field(xpfc_ctx* ctx, int s, double* host_real);
+long long xpfc_inst_current_step(const xpfc_ctx* ctx, int s);
+int xpfc_inst_num_samples(const xpfc_ctx* ctx, int s);
+int xpfc_inst_get_samples(const xpfc_ctx* ctx, int s, xpfc_sample* out, int max);
+void xpfc_inst_clear_samples(xpfc_ctx* ctx, int s);
+int xpfc_inst_fe_calc(xpfc_ctx* ctx, int s, double* F_ref, double* f_mean);
+int xpfc_inst_avg_calc(xpfc_ctx* ctx, int s, double* mean);
+int xpfc_inst_sse_calc(xpfc_ctx* ctx, int s, double* rmse);
+int xpfc_inst_write_vtk(xpfc_ctx* ctx, int s, const char* path);
+
 /* ---- errors ----------------------------------------------------------------- */
 const char* xpfc_strerror(int status);
 const char* xpfc_last_error(void);   /* detail of the last failure on this thread */

@@ -126,3 +126,17 @@ def test_read_input_extensions(tmp_path):
     assert p.n == (256, 256, 128) and p.L[2] == 16.0
     assert p.precision == X.XPFC_FP32_MIXED and p.dcf_mode == X.XPFC_DCF_MAX
     assert len(p.peaks) == 1 and p.peaks[0][2] == 1.0
+
+
+def test_batch_rejects_mismatched_grid():
+    """xpfc_create_batch checks the instances' grids before touching a device."""
+    from cases import make
+    _, p = make((32, 32))
+    _, q = make((64, 32))
+    with pytest.raises(X.XPFCError):
+        X.BatchContext([p, q])
+    _, r = make((32, 32), prec=X.XPFC_FP32_MIXED)
+    with pytest.raises(X.XPFCError):
+        X.BatchContext([p, r])
+    with pytest.raises(ValueError):
+        X.BatchContext([])

@@ -111,6 +111,23 @@ def load_library(path: Optional[str] = None):
         "xpfc_fft_c2r": (i, [vp, vp, vp]),
         "xpfc_write_vtk": (i, [vp, ct.c_char_p]),
         "xpfc_read_vtk": (i, [ct.c_char_p, P(i), P(i), P(d), P(d), ll]),
+        "xpfc_create_batch": (i, [P(_Params), i, P(vp)]),
+        "xpfc_num_instances": (i, [vp]),
+        "xpfc_set_active": (i, [vp, P(i), i]),
+        "xpfc_inst_set_params": (i, [vp, i, P(_Params)]),
+        "xpfc_inst_reset": (i, [vp, i]),
+        "xpfc_inst_initialize": (i, [vp, i]),
+        "xpfc_inst_initialize_noise": (i, [vp, i, d, ct.c_ulonglong]),
+        "xpfc_inst_set_field": (i, [vp, i, P(d)]),
+        "xpfc_inst_get_field": (i, [vp, i, P(d)]),
+        "xpfc_inst_current_step": (ll, [vp, i]),
+        "xpfc_inst_num_samples": (i, [vp, i]),
+        "xpfc_inst_get_samples": (i, [vp, i, P(_Sample), i]),
+        "xpfc_inst_clear_samples": (None, [vp, i]),
+        "xpfc_inst_fe_calc": (i, [vp, i, P(d), P(d)]),
+        "xpfc_inst_avg_calc": (i, [vp, i, P(d)]),
+        "xpfc_inst_sse_calc": (i, [vp, i, P(d)]),
+        "xpfc_inst_write_vtk": (i, [vp, i, ct.c_char_p]),
         "xpfc_strerror": (ct.c_char_p, [i]),
         "xpfc_last_error": (ct.c_char_p, []),
     }
@@ -489,3 +506,108 @@ class Context:
         d = DeviceBuffer(self, int(np.prod(self.shape)) * np.dtype(self.real_dtype).itemsize)
         _check(lib().xpfc_fft_c2r(self.handle, s.ptr, d.ptr), "xpfc_fft_c2r")
         return d.download(self.real_dtype, self.shape)
+
+
+class BatchContext:
+    """Several simulations of one grid in one context (xpfc_create_batch).
+
+    Every instance has its own physics, field, step counter and sample log;
+    step() advances the active instances with one launch per pass.  No
+    counterpart in the reference (one simulation per process, main.cu:20-435).
+    """
+
+    def __init__(self, params: Sequence[Params]):
+        if not params:
+            raise ValueError("at least one instance")
+        self.params = list(params)
+        arr = (_Params * len(self.params))(*[q.to_c() for q in self.params])
+        h = ct.c_void_p()
+        _check(lib().xpfc_create_batch(arr, len(self.params), ct.byref(h)), "xpfc_create_batch")
+        self.handle = h
+
+    close = Context.close
+    __del__ = Context.__del__
+    __enter__ = Context.__enter__
+    __exit__ = Context.__exit__
+    step = Context.step
+    time_steps = Context.time_steps
+    time_passes = Context.time_passes
+    autotune = Context.autotune
+    set_variants = Context.set_variants
+    variants = Context.variants
+    set_graphs = Context.set_graphs
+    sync = Context.sync
+
+    @property
+    def ninst(self) -> int:
+        return int(lib().xpfc_num_instances(self.handle))
+
+    @property
+    def shape(self):
+        return tuple(self.params[0].n)
+
+    def set_active(self, inst: Optional[Sequence[int]] = None):
+        if inst is None:
+            _check(lib().xpfc_set_active(self.handle, None, 0), "xpfc_set_active")
+        else:
+            a = (ct.c_int * max(len(inst), 1))(*inst)
+            _check(lib().xpfc_set_active(self.handle, a, len(inst)), "xpfc_set_active")
+
+    def set_params(self, s: int, p: Params):
+        c = p.to_c()
+        _check(lib().xpfc_inst_set_params(self.handle, s, ct.byref(c)), "xpfc_inst_set_params")
+        self.params[s] = p
+
+    def reset(self, s: int):
+        _check(lib().xpfc_inst_reset(self.handle, s), "xpfc_inst_reset")
+
+    def initialize(self, s: int):
+        _check(lib().xpfc_inst_initialize(self.handle, s), "xpfc_inst_initialize")
+
+    def initialize_noise(self, s: int, amplitude: float, seed: int):
+        _check(lib().xpfc_inst_initialize_noise(self.handle, s, float(amplitude), int(seed)),
+               "xpfc_inst_initialize_noise")
+
+    def set_field(self, s: int, psi: np.ndarray):
+        a = np.ascontiguousarray(psi, dtype=np.float64)
+        assert a.shape == self.shape, (a.shape, self.shape)
+        _check(lib().xpfc_inst_set_field(self.handle, s, _dptr(a)), "xpfc_inst_set_field")
+
+    def get_field(self, s: int) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.float64)
+        _check(lib().xpfc_inst_get_field(self.handle, s, _dptr(out)), "xpfc_inst_get_field")
+        return out
+
+    def current_step(self, s: int) -> int:
+        return int(lib().xpfc_inst_current_step(self.handle, s))
+
+    def samples(self, s: int) -> List[Sample]:
+        n = lib().xpfc_inst_num_samples(self.handle, s)
+        arr = (_Sample * max(n, 1))()
+        got = lib().xpfc_inst_get_samples(self.handle, s, arr, n)
+        if got < 0:
+            _check(got, "xpfc_inst_get_samples")
+        return [Sample(int(r.step), bool(r.has_fe), bool(r.has_sse), r.F_ref, r.f_mean, r.mean,
+                       r.rmse) for r in arr[:got]]
+
+    def clear_samples(self, s: int):
+        lib().xpfc_inst_clear_samples(self.handle, s)
+
+    def fe_calc(self, s: int) -> Tuple[float, float]:
+        F, fm = ct.c_double(), ct.c_double()
+        _check(lib().xpfc_inst_fe_calc(self.handle, s, ct.byref(F), ct.byref(fm)),
+               "xpfc_inst_fe_calc")
+        return F.value, fm.value
+
+    def avg_calc(self, s: int) -> float:
+        m = ct.c_double()
+        _check(lib().xpfc_inst_avg_calc(self.handle, s, ct.byref(m)), "xpfc_inst_avg_calc")
+        return m.value
+
+    def sse_calc(self, s: int) -> float:
+        r = ct.c_double()
+        _check(lib().xpfc_inst_sse_calc(self.handle, s, ct.byref(r)), "xpfc_inst_sse_calc")
+        return r.value
+
+    def write_vtk(self, s: int, path: str):
+        _check(lib().xpfc_inst_write_vtk(self.handle, s, path.encode()), "xpfc_inst_write_vtk")

@@ -24,7 +24,7 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_host.cpp"]
+SOURCES = ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp"]
 
 
 def _hipcc() -> str:

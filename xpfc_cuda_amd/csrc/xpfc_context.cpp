@@ -1,0 +1,1174 @@
+// The XPFC context: device fields of K simulations of one grid, twiddle and
+// instance tables, the step driver (hipGraph step blocks), the autotuner,
+// diagnostics, field I/O and the operator-level entry points.
+//
+// A context holds K >= 1 instances (xpfc_create: K = 1).  Instance s keeps its
+// spectra at s * stride complex elements from the buffer bases; its physics
+// (Phys, ZArgs) sits in a device table read by the kernels, so one launch per
+// pass advances every active instance (blockIdx.y -> instance slot).
+#include "xpfc_host.hpp"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+using namespace xpfc;
+
+namespace {
+
+constexpr int SUM_ROWS = 1024;  // sampled steps held on the device per read-back
+constexpr int DIAG_ROW = SUM_ROWS;  // extra row for on-demand diagnostics
+constexpr int GRAPH_BLOCK = 50;  // steps per captured step block
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+template <typename T>
+std::vector<T> twiddle_table(int L, int denom) {
+  // tw[m] = exp(-2 pi i m / denom), m < L, evaluated in long double
+  std::vector<T> t(2 * (size_t)L);
+  const long double two_pi = 6.283185307179586476925286766559005768L;
+  for (int m = 0; m < L; ++m) {
+    const long double ang = two_pi * (long double)m / (long double)denom;
+    t[2 * m] = (T)std::cos(ang);
+    t[2 * m + 1] = (T)(-std::sin(ang));
+  }
+  return t;
+}
+
+int upload_twiddles(xpfc_ctx* c, void** dst, int L, int denom) {
+  const size_t bytes = (size_t)L * c->csize;
+  HIPCHK(hipMalloc(dst, bytes));
+  if (c->prec == 1) {
+    auto t = twiddle_table<float>(L, denom);
+    HIPCHK(hipMemcpy(*dst, t.data(), bytes, hipMemcpyHostToDevice));
+  } else {
+    auto t = twiddle_table<double>(L, denom);
+    HIPCHK(hipMemcpy(*dst, t.data(), bytes, hipMemcpyHostToDevice));
+  }
+  return XPFC_OK;
+}
+
+// widest valid tile (16 columns FP32 / 8 FP64 keeps every row access 128 bytes)
+int default_tc(int prec, int L, const Segs& s) {
+  const int pref = prec == 1 ? 16 : 8;
+  for (int tc = pref; tc >= 4; tc /= 2)
+    if (strided_blocks(prec, L, tc, s) >= 0) return tc;
+  return -1;
+}
+
+// Captured step blocks bake in buffer pointers, pass variants and the active
+// instance count: drop them whenever one of those changes.
+void drop_graphs(xpfc_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
+  c->graphs.clear();
+}
+
+void free_all(xpfc_ctx* c) {
+  drop_graphs(c);
+  void* ptrs[] = {c->psi, c->work, c->aux, c->tw0, c->tw1, c->twz, c->twp, c->px,
+                  c->pz, c->sums, c->d_ia, c->d_ids, c->d_active};
+  for (void* q : ptrs)
+    if (q) (void)hipFree(q);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+}
+
+bool same_grid(const xpfc_params& a, const xpfc_params& b) {
+  if (a.dim != b.dim || a.precision != b.precision) return false;
+  for (int d = 0; d < a.dim; ++d)
+    if (a.n[d] != b.n[d] || a.L[d] != b.L[d]) return false;
+  return true;
+}
+
+// physics of one instance -> its kernel table entry
+int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
+  if (p.npeaks < 1 || p.npeaks > XPFC_MAX_PEAKS) return fail(XPFC_ERR_ARG, "npeaks out of range");
+  if (!(p.dt > 0.0)) return fail(XPFC_ERR_ARG, "dt must be positive");
+  in->p = p;
+  Phys& ph = in->ia.ph;
+  ph = Phys{};
+  ph.dtM = p.dt * p.Mo;
+  ph.one_m_eps = 1.0 - p.eps;
+  ph.inv_dt = 1.0 / p.dt;
+  ph.Mres = p.int_Mo_residual ? (double)(int)p.Mo : p.Mo;
+  ph.npk = p.npeaks;
+  ph.dcf_max = p.dcf_mode == XPFC_DCF_MAX;
+  // a peak term below exp(-ecut) is dropped: 1e-10 (FP32 fields) / 4e-18 (FP64)
+  // of the denominator, which is >= 1
+  ph.ecut = c->prec == 1 ? 23.0 : 40.0;
+  double kc = 0.0;
+  for (int i = 0; i < p.npeaks; ++i) {
+    ph.pk[i] = p.peak_k[i];
+    ph.inv2a2[i] = 1.0 / (2.0 * p.peak_alpha[i] * p.peak_alpha[i]);
+    ph.amp[i] = std::exp(-(p.sigma * p.peak_k[i]) * (p.sigma * p.peak_k[i]) / (2.0 * p.peak_rho_beta[i]));
+    kc = std::max(kc, p.peak_k[i] + std::sqrt(ph.ecut / ph.inv2a2[i]));
+  }
+  ph.kcut2 = kc * kc;
+  in->c2zero = c2hat_host(0.0, p);
+  in->ia.za.a = p.a;
+  in->ia.za.b = p.b;
+  in->ia.za.scale = 1.0 / (double)c->N;
+  in->ia.za.mean = in->mean;
+  return XPFC_OK;
+}
+
+int upload_inst(xpfc_ctx* c, int s) {
+  c->inst[s].ia.za.mean = c->inst[s].mean;
+  HIPCHK(hipMemcpyAsync(c->d_ia + s, &c->inst[s].ia, sizeof(InstArgs), hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+
+Batch bt_of(const xpfc_ctx* c, const int* slots) {
+  return Batch{slots, c->d_ia, c->stride, (long long)c->npx_cap * XPART,
+               (long long)c->npz_cap * ZPART};
+}
+Batch bt_active(const xpfc_ctx* c) { return bt_of(c, c->d_active); }
+Batch bt_one(const xpfc_ctx* c, int s) { return bt_of(c, c->d_ids + s); }
+
+void* ny_of(const xpfc_ctx* c, void* spec) { return (char*)spec + (size_t)c->nmain * c->csize; }
+void* inst_ptr(const xpfc_ctx* c, void* base, int s) {
+  return (char*)base + (size_t)s * c->stride * c->csize;
+}
+
+int check_inst(const xpfc_ctx* c, int s) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (s < 0 || s >= c->K) return fail(XPFC_ERR_ARG, "instance %d out of range [0, %d)", s, c->K);
+  return XPFC_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// creation
+// ---------------------------------------------------------------------------
+extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
+  if (!ps || !out || K < 1) return fail(XPFC_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  xpfc_params p = ps[0];
+  if (p.dim != 2 && p.dim != 3) return fail(XPFC_ERR_ARG, "dim must be 2 or 3");
+  if (p.dim == 2) {
+    p.n[2] = 1;
+    p.L[2] = 1.0;
+  }
+  const int nd = p.dim;
+  for (int d = 0; d < nd; ++d) {
+    if (!is_pow2(p.n[d]) || p.n[d] < 8 || p.n[d] > 1024)
+      return fail(XPFC_ERR_ARG, "grid extent n[%d]=%d must be a power of two in [8, 1024]", d, p.n[d]);
+    if (!(p.L[d] > 0.0)) return fail(XPFC_ERR_ARG, "box length L[%d] must be positive", d);
+  }
+  if (p.precision != XPFC_FP64 && p.precision != XPFC_FP32_MIXED)
+    return fail(XPFC_ERR_ARG, "bad precision");
+  for (int s = 1; s < K; ++s) {
+    xpfc_params q = ps[s];
+    if (q.dim == 2) {
+      q.n[2] = 1;
+      q.L[2] = 1.0;
+    }
+    if (!same_grid(p, q)) return fail(XPFC_ERR_ARG, "instance %d: grid/precision differ from instance 0", s);
+  }
+
+  xpfc_ctx* c = new (std::nothrow) xpfc_ctx();
+  if (!c) return fail(XPFC_ERR_NOMEM, "host allocation failed");
+  c->p = p;
+  c->K = K;
+  c->prec = p.precision == XPFC_FP32_MIXED ? 1 : 0;
+  // axis roles: 2-D (Mx, My) -> (n0, n1, n2) = (Mx, 1, My); 3-D (Mx, My, Mz)
+  double L0, L1, L2;
+  if (nd == 2) {
+    c->n0 = p.n[0]; c->n1 = 1; c->n2 = p.n[1];
+    L0 = p.L[0]; L1 = 1.0; L2 = p.L[1];
+  } else {
+    c->n0 = p.n[0]; c->n1 = p.n[1]; c->n2 = p.n[2];
+    L0 = p.L[0]; L1 = p.L[1]; L2 = p.L[2];
+  }
+  c->H = c->n2 / 2;
+  c->N = (long long)c->n0 * c->n1 * c->n2;
+  c->nmain = (long long)c->n0 * c->n1 * c->H;
+  c->nny = (long long)c->n0 * c->n1;
+  c->stride = c->nmain + c->nny;
+  c->csize = c->prec == 1 ? 8 : 16;
+  c->rsize = c->prec == 1 ? 4 : 8;
+
+  int rc = XPFC_OK;
+  auto bail = [&](int r) {
+    free_all(c);
+    delete c;
+    return r;
+  };
+  if (hipSetDevice(p.device) != hipSuccess) return bail(fail(XPFC_ERR_DEVICE, "hipSetDevice(%d) failed", p.device));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
+  const size_t sbytes = (size_t)K * c->stride * c->csize;
+  if (hipMalloc(&c->psi, sbytes) != hipSuccess || hipMalloc(&c->work, sbytes) != hipSuccess ||
+      hipMalloc(&c->aux, sbytes) != hipSuccess)
+    return bail(fail(XPFC_ERR_NOMEM, "device allocation of %zu bytes x3 failed", sbytes));
+  if ((rc = upload_twiddles(c, &c->tw0, c->n0, c->n0)) != XPFC_OK) return bail(rc);
+  if ((rc = upload_twiddles(c, &c->tw1, c->n1, c->n1)) != XPFC_OK) return bail(rc);
+  if ((rc = upload_twiddles(c, &c->twz, c->H, c->H)) != XPFC_OK) return bail(rc);
+  if ((rc = upload_twiddles(c, &c->twp, c->H, c->n2)) != XPFC_OK) return bail(rc);
+
+  // strided segments: main half spectrum and the Nyquist plane
+  c->segX.s[0] = Seg{0, 0, (int)(c->n1 * c->H), 1, (int)(c->n1 * c->H), 1, 0, 0};
+  c->segX.s[1] = Seg{c->nmain, 0, c->n1, 1, c->n1, 1, 0, 1};
+  c->segY.s[0] = Seg{0, (long long)c->n1 * c->H, c->H, 1, c->H, c->n0, 0, 0};
+  c->segY.s[1] = Seg{c->nmain, 0, 1, c->n1, c->n0, 1, 0, 1};
+  c->tc_x = default_tc(c->prec, c->n0, c->segX);
+  c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, c->segY) : 0;
+  if (c->tc_x <= 0 || (c->n1 > 1 && c->tc_y <= 0)) return bail(fail(XPFC_ERR_ARG, "unsupported grid"));
+  c->ntz = 256;
+  // partial buffers sized for the largest variant (narrowest tile, smallest block)
+  c->npx_cap = 4096;
+  for (int tc : {4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, c->segX));
+  c->npz_cap = 1;
+  for (int nz : {64, 128, 256, 512})
+    c->npz_cap = std::max(c->npz_cap, zpass_blocks(c->prec, c->H, nz, (long long)c->n0 * c->n1));
+  if (hipMalloc((void**)&c->px, (size_t)K * c->npx_cap * XPART * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->pz, (size_t)K * c->npz_cap * ZPART * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->sums, (size_t)(SUM_ROWS + 1) * K * NSUMS * sizeof(double)) != hipSuccess ||
+      hipMalloc((void**)&c->d_ia, (size_t)(K + 1) * sizeof(InstArgs)) != hipSuccess ||
+      hipMalloc((void**)&c->d_ids, (size_t)(K + 1) * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_active, (size_t)K * sizeof(int)) != hipSuccess)
+    return bail(fail(XPFC_ERR_NOMEM, "table allocation failed"));
+
+  c->g.n0 = c->n0;
+  c->g.n1 = c->n1;
+  c->g.H = c->H;
+  c->g.dk0 = 2.0 * M_PI / L0;
+  c->g.dk1 = nd == 3 ? 2.0 * M_PI / L1 : 0.0;
+  c->g.dk2 = 2.0 * M_PI / L2;
+
+  c->inst.resize(K);
+  std::vector<InstArgs> ia(K + 1);
+  for (int s = 0; s < K; ++s) {
+    c->inst[s].mean = ps[s].psi0;
+    if ((rc = build_inst(c, ps[s], &c->inst[s])) != XPFC_OK) return bail(rc);
+    ia[s] = c->inst[s].ia;
+  }
+  ia[K] = ia[0];
+  std::vector<int> ids(K + 1);
+  for (int s = 0; s <= K; ++s) ids[s] = s;
+  if (hipMemcpy(c->d_ia, ia.data(), ia.size() * sizeof(InstArgs), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_ids, ids.data(), ids.size() * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(c->d_active, ids.data(), K * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    return bail(fail(XPFC_ERR_DEVICE, "table upload failed"));
+  c->active.assign(ids.begin(), ids.begin() + K);
+  const char* eg = std::getenv("XPFC_GRAPHS");
+  c->use_graphs = !(eg && eg[0] == '0');
+  *out = c;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_create(const xpfc_params* p, xpfc_ctx** out) { return xpfc_create_batch(p, 1, out); }
+
+extern "C" void xpfc_destroy(xpfc_ctx* c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  free_all(c);
+  delete c;
+}
+
+extern "C" int xpfc_get_params(const xpfc_ctx* c, xpfc_params* out) {
+  if (!c || !out) return fail(XPFC_ERR_ARG, "null argument");
+  *out = c->inst[0].p;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_num_instances(const xpfc_ctx* c) { return c ? c->K : 0; }
+extern "C" long long xpfc_num_cells(const xpfc_ctx* c) { return c ? c->N : 0; }
+extern "C" size_t xpfc_real_bytes(const xpfc_ctx* c) { return c ? (size_t)c->N * c->rsize : 0; }
+extern "C" size_t xpfc_spectrum_bytes(const xpfc_ctx* c) {
+  return c ? (size_t)c->n0 * c->n1 * (c->H + 1) * c->csize : 0;
+}
+
+extern "C" int xpfc_inst_set_params(xpfc_ctx* c, int s, const xpfc_params* p) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!p) return fail(XPFC_ERR_ARG, "null params");
+  xpfc_params q = *p;
+  if (q.dim == 2) {
+    q.n[2] = 1;
+    q.L[2] = 1.0;
+  }
+  if (!same_grid(c->p, q)) return fail(XPFC_ERR_ARG, "grid/precision differ from the context's");
+  if ((rc = build_inst(c, q, &c->inst[s])) != XPFC_OK) return rc;
+  return upload_inst(c, s);
+}
+
+extern "C" int xpfc_inst_reset(xpfc_ctx* c, int s) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  XInst& in = c->inst[s];
+  in.step = 0;
+  in.samples.clear();
+  in.have_field = false;
+  in.have_rmse = false;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  std::vector<int> a;
+  if (!slots) {
+    for (int s = 0; s < c->K; ++s) a.push_back(s);
+  } else {
+    if (n < 0 || n > c->K) return fail(XPFC_ERR_ARG, "bad active count");
+    std::vector<char> seen(c->K, 0);
+    for (int i = 0; i < n; ++i) {
+      if (slots[i] < 0 || slots[i] >= c->K || seen[slots[i]])
+        return fail(XPFC_ERR_ARG, "bad or repeated instance %d", slots[i]);
+      seen[slots[i]] = 1;
+      a.push_back(slots[i]);
+    }
+  }
+  if (!a.empty()) {
+    HIPCHK(hipMemcpyAsync(c->d_active, a.data(), a.size() * sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  c->active = a;
+  drop_graphs(c);
+  return XPFC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// transforms used for field I/O and diagnostics (one instance)
+// ---------------------------------------------------------------------------
+namespace {
+
+int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->stream));
+  if (do_x)
+    LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, -1, c->tw0, bt, ny, c->stream));
+  return XPFC_OK;
+}
+
+int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
+  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, +1, c->tw0, bt, ny, c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->stream));
+  return XPFC_OK;
+}
+
+// instance s: psi state spectrum -> real field (working precision) in its aux
+// region, optional real-space sums into its Z partials
+int psi_to_real(xpfc_ctx* c, int s, bool sums) {
+  const size_t sbytes = (size_t)c->stride * c->csize;
+  HIPCHK(hipMemcpyAsync(inst_ptr(c, c->aux, s), inst_ptr(c, c->psi, s), sbytes,
+                        hipMemcpyDeviceToDevice, c->stream));
+  const Batch bt = bt_one(c, s);
+  int rc = inverse_xy(c, c->aux, bt, 1);
+  if (rc) return rc;
+  LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), c->aux, nullptr,
+                    (long long)c->n0 * c->n1, c->H, 256, bt, 1, c->twz, c->twp, c->pz, sums,
+                    c->stream));
+  return XPFC_OK;
+}
+
+// fluctuation (host FP64) -> instance s's aux region (working precision)
+int upload_fluct(xpfc_ctx* c, int s, const double* host, double mean) {
+  const size_t n = (size_t)c->N;
+  void* dst = inst_ptr(c, c->aux, s);
+  if (c->prec == 1) {
+    std::vector<float> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = (float)(host[i] - mean);
+    HIPCHK(hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  } else {
+    std::vector<double> f(n);
+    for (size_t i = 0; i < n; ++i) f[i] = host[i] - mean;
+    HIPCHK(hipMemcpyAsync(dst, f.data(), n * 8, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return XPFC_OK;
+}
+
+}  // namespace
+
+extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!host) return fail(XPFC_ERR_ARG, "null field");
+  // Avg_calc.cu:3-18 mean, FP64, row-major order
+  double sum = 0.0;
+  for (long long i = 0; i < c->N; ++i) sum += host[i];
+  const double mean = sum / (double)c->N;
+  if (!std::isfinite(mean)) return fail(XPFC_ERR_NONFINITE, "field contains NaN/Inf");
+  c->inst[s].mean = mean;
+  if ((rc = upload_inst(c, s))) return rc;
+  if ((rc = upload_fluct(c, s, host, mean))) return rc;
+  const long long nl = (long long)c->n0 * c->n1;
+  const Batch bt = bt_one(c, s);
+  // instance s's constants, pointers already at instance s's regions
+  const Batch here{c->d_ids + s, c->d_ia, 0, 0, 0};
+  void* aux = inst_ptr(c, c->aux, s);
+  void* psi = inst_ptr(c, c->psi, s);
+  void* work = inst_ptr(c, c->work, s);
+  // psi_k = FFT(psi - mean): the k = 0 slot is then zero by construction
+  LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, aux, nullptr, psi, ny_of(c, psi), nl, c->H, 256, here, 1,
+                    c->twz, c->twp, nullptr, false, c->stream));
+  if ((rc = forward_xy(c, c->psi, bt, 1, true))) return rc;
+  HIPCHK(hipMemsetAsync(psi, 0, c->csize, c->stream));
+  // work = FFT_yz(N(psi)): the nonlinear term, all axes but axis 0
+  LCHK(launch_zpass(c->prec, Z_R2C_NL, aux, nullptr, work, ny_of(c, work), nl, c->H, 256, here, 1,
+                    c->twz, c->twp, nullptr, false, c->stream));
+  if ((rc = forward_xy(c, c->work, bt, 1, false))) return rc;
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->inst[s].have_field = true;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_inst_get_field(xpfc_ctx* c, int s, double* host) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!host) return fail(XPFC_ERR_ARG, "null field");
+  if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
+  if ((rc = psi_to_real(c, s, false))) return rc;
+  const size_t n = (size_t)c->N;
+  const void* src = inst_ptr(c, c->aux, s);
+  if (c->prec == 1) {
+    std::vector<float> f(n);
+    HIPCHK(hipMemcpyAsync(f.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    for (size_t i = 0; i < n; ++i) host[i] = (double)f[i];
+  } else {
+    HIPCHK(hipMemcpyAsync(host, src, n * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_field(xpfc_ctx* c, const double* host) { return xpfc_inst_set_field(c, 0, host); }
+extern "C" int xpfc_get_field(xpfc_ctx* c, double* host) { return xpfc_inst_get_field(c, 0, host); }
+
+// ---------------------------------------------------------------------------
+// initial conditions (initialize.cu:95-202)
+// ---------------------------------------------------------------------------
+extern "C" int xpfc_inst_initialize(xpfc_ctx* c, int s) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  const xpfc_params& p = c->inst[s].p;
+  std::vector<double> psi((size_t)c->N, p.psi0);
+  if (p.iconfig == 10) {
+    int nx = 0, ny = 0;
+    double sp[3];
+    std::vector<double> v;
+    if ((rc = read_vtk_file("initial_density_field.vtk", &nx, &ny, sp, &v))) return rc;
+    if ((long long)v.size() != c->N) return fail(XPFC_ERR_ARG, "VTK field size does not match the grid");
+    psi = v;  // read_input.cu:208-218 fills psi[i*My + j] in file order
+  } else if (p.dim == 2) {
+    const int Mx = p.n[0], My = p.n[1];
+    const auto x = linspace_ref(0.0, p.L[0], Mx);
+    const auto y = linspace_ref(0.0, p.L[1], My);
+    const double a_tri = 2.0 / std::sqrt(3.0), a_sq = 1.0;
+    const double k_sq = 2.0 * M_PI / a_sq, k_tri = 2.0 * M_PI / a_tri;
+    for (int i = 0; i < Mx; ++i)
+      for (int j = 0; j < My; ++j) {
+        const double X = x[i], Y = y[j];
+        double v;
+        if (p.iconfig == 1)  // initialize.cu:96-99
+          v = p.A_10 * ((3.0 * std::sqrt(2.0) - 2.0) * std::cos(k_sq * Y) - 2.0 * std::cos(k_sq * X)) -
+              p.A_11 * std::cos(k_sq * X) * std::cos(k_sq * Y);
+        else if (p.iconfig == 0)  // initialize.cu:116-118
+          v = p.A_tri * (std::cos(k_tri * X) / 2.0 +
+                         std::cos(k_tri * X / 2.0) * std::cos(std::sqrt(3.0) * k_tri * Y / 2.0));
+        else
+          v = 0.0;
+        psi[(size_t)i * My + j] = v + p.psi0;
+      }
+    const double r = p.iconfig == 1 ? p.roa * a_sq : (p.iconfig == 0 ? p.roa * a_tri : 0.0);
+    if (p.iseed == 1) {  // initialize.cu:158-191
+      for (int i = 0; i < Mx; ++i)
+        for (int j = 0; j < My; ++j) {
+          double& v = psi[(size_t)i * My + j];
+          const double dx = x[i] - p.h, dy = y[j] - p.k;
+          if (p.ishape == 0) {
+            if (std::sqrt(dx * dx + dy * dy) > r) v = p.psi0;
+          } else {
+            const double gs = std::exp(-(dx * dx + dy * dy) / p.roa);
+            v = (v - p.psi0) * gs + p.psi0 * (1.0 - gs);
+          }
+        }
+    }
+  } else if (p.iconfig != 2) {
+    return fail(XPFC_ERR_ARG, "3-D lattice initial conditions are not defined; use liquid (2) or noise");
+  }
+  fix_mean(psi, p.psi0);
+  return xpfc_inst_set_field(c, s, psi.data());
+}
+
+extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, unsigned long long seed) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  std::vector<double> psi((size_t)c->N);
+  if ((rc = xpfc_randn(seed, 0, c->N, psi.data()))) return rc;
+  const double psi0 = c->inst[s].p.psi0;
+  for (double& v : psi) v = psi0 + amplitude * v;
+  fix_mean(psi, psi0);
+  return xpfc_inst_set_field(c, s, psi.data());
+}
+
+extern "C" int xpfc_initialize(xpfc_ctx* c) { return xpfc_inst_initialize(c, 0); }
+extern "C" int xpfc_initialize_noise(xpfc_ctx* c, double amplitude, unsigned long long seed) {
+  return xpfc_inst_initialize_noise(c, 0, amplitude, seed);
+}
+
+// ---------------------------------------------------------------------------
+// the step driver
+// ---------------------------------------------------------------------------
+namespace {
+
+bool inst_samples(const XInst& in, long long n) {
+  const long long ts = in.p.t_sample;
+  return ts > 0 && (n % ts == 0 || (n + 1) % ts == 0);
+}
+
+// The launches of one step (four in 3-D, two in 2-D) over every active
+// instance.  With `ev`, an event is recorded after each pass for timing.
+int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr) {
+  const long long nl = (long long)c->n0 * c->n1;
+  const Batch bt = bt_active(c);
+  const int ny = (int)c->active.size();
+  const int nbx = launch_xstep(c->prec, c->work, c->psi, c->segX, c->n0, c->tc_x, c->tw0, c->g, bt, ny,
+                               c->px, sample, c->stream);
+  LCHK(nbx);
+  if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
+  LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl, c->H,
+                    c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
+  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->stream));
+  if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
+  if (sample)
+    LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->prec, c->H, c->ntz, nl), bt, ny,
+                         c->sums + (size_t)row * c->K * NSUMS, c->stream));
+  return XPFC_OK;
+}
+
+// Unsampled steps run as replays of captured hipGraphs of up to GRAPH_BLOCK
+// steps (one graph per run length, cached per context).  The captured work is
+// the same launch sequence as an eager step, so the fields are identical.
+int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
+  for (auto& g : c->graphs)
+    if (g.first == len) {
+      *out = g.second;
+      return XPFC_OK;
+    }
+  if (c->graphs.size() >= 8) drop_graphs(c);
+  hipGraph_t graph = nullptr;
+  HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  int rc = XPFC_OK;
+  for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = launch_one_step(c, false, 0);
+  const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+  if (rc != XPFC_OK || ec != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return rc != XPFC_OK ? rc : fail(XPFC_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(ec));
+  }
+  hipGraphExec_t ex = nullptr;
+  const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+  c->graphs.emplace_back(len, ex);
+  *out = ex;
+  return XPFC_OK;
+}
+
+void advance(xpfc_ctx* c, long long n) {
+  for (int s : c->active) c->inst[s].step += n;
+}
+
+int run_plain_steps(xpfc_ctx* c, long long run) {
+  while (run > 0) {
+    if (!c->use_graphs) {
+      int rc = launch_one_step(c, false, 0);
+      if (rc) return rc;
+      advance(c, 1);
+      --run;
+      continue;
+    }
+    const int len = (int)std::min<long long>(run, GRAPH_BLOCK);
+    hipGraphExec_t ex = nullptr;
+    int rc = plain_graph(c, len, &ex);
+    if (rc) return rc;
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+    advance(c, len);
+    run -= len;
+  }
+  return XPFC_OK;
+}
+
+double ff_factor(const xpfc_params& p) {  // initialize.cu:63
+  double ff = 1.0;
+  for (int d = 0; d < p.dim; ++d) ff *= p.L[d] / (double)(p.n[d] - 1);
+  return ff;
+}
+
+// sum f = 1/2 s2 - a/3 s3 + b/4 s4 - 1/2 sum_x psi (C2*psi)  (FE_calc.cu:64), with
+// sum_x psi (C2*psi) = (1/N) sum_k c2 |psi_k|^2 by Parseval, k = 0 carried in FP64
+double fe_sum(const xpfc_ctx* c, const XInst& in, double conv, double s2, double s3, double s4) {
+  const double N = (double)c->N;
+  const double conv_x = (conv + in.c2zero * (N * in.mean) * (N * in.mean)) / N;
+  return 0.5 * s2 - (in.p.a / 3.0) * s3 + (in.p.b / 4.0) * s4 - 0.5 * conv_x;
+}
+
+void add_fe_record(xpfc_ctx* c, XInst& in, long long step, const double* s) {
+  const double sf = fe_sum(c, in, s[0], s[3], s[4], s[5]);
+  const double N = (double)c->N;
+  if (!in.samples.empty() && in.samples.back().step == step && !in.samples.back().has_fe) {
+    xpfc_sample& b = in.samples.back();
+    b.has_fe = 1;
+    b.F_ref = ff_factor(in.p) * sf;
+    b.f_mean = sf / N;
+    b.mean = s[2] / N;
+    return;
+  }
+  xpfc_sample r{};
+  r.step = step;
+  r.has_fe = 1;
+  r.F_ref = ff_factor(in.p) * sf;
+  r.f_mean = sf / N;
+  r.mean = s[2] / N;
+  r.rmse = NAN;
+  in.samples.push_back(r);
+}
+
+void add_sse_record(xpfc_ctx* c, XInst& in, long long step, double sse) {
+  const double rmse = std::sqrt(sse) / (double)c->N;  // sqrt(sum_x R^2 / N), Parseval
+  in.last_rmse = rmse;
+  in.have_rmse = true;
+  if (!in.samples.empty() && in.samples.back().step == step) {
+    in.samples.back().has_sse = 1;
+    in.samples.back().rmse = rmse;
+    return;
+  }
+  xpfc_sample r{};
+  r.step = step;
+  r.has_sse = 1;
+  r.rmse = rmse;
+  r.F_ref = r.f_mean = r.mean = NAN;
+  in.samples.push_back(r);
+}
+
+int drain(xpfc_ctx* c) {
+  if (c->pending.empty()) return XPFC_OK;
+  const int rows = (int)c->pending.size();
+  std::vector<double> h((size_t)rows * c->K * NSUMS);
+  HIPCHK(hipMemcpyAsync(h.data(), c->sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  bool bad = false;
+  for (const XPending& pr : c->pending)
+    for (const auto& w : pr.who) {
+      XInst& in = c->inst[w.first];
+      const double* s = &h[((size_t)pr.row * c->K + w.first) * NSUMS];
+      for (int k = 0; k < NSUMS; ++k) bad |= !std::isfinite(s[k]);
+      const long long n = w.second, ts = in.p.t_sample;
+      if (n % ts == 0) add_sse_record(c, in, n, s[1]);
+      if ((n + 1) % ts == 0) add_fe_record(c, in, n + 1, s);
+    }
+  c->pending.clear();
+  if (bad) return fail(XPFC_ERR_NONFINITE, "non-finite value in a sampled reduction");
+  return XPFC_OK;
+}
+
+int fe_now(xpfc_ctx* c, int s, double* F, double* fm, double* mean_rs) {
+  int nb = 0;
+  const Batch bt = bt_one(c, s);
+  LCHK(launch_kspace_sums(c->prec, c->psi, c->nmain, c->nny, c->g, bt, c->px, &nb, c->stream));
+  int rc = psi_to_real(c, s, true);
+  if (rc) return rc;
+  double* row = c->sums + (size_t)DIAG_ROW * c->K * NSUMS;
+  LCHK(launch_finalize(c->px, nb, c->pz, zpass_blocks(c->prec, c->H, 256, (long long)c->n0 * c->n1), bt,
+                       1, row, c->stream));
+  double h[NSUMS];
+  HIPCHK(hipMemcpyAsync(h, row + (size_t)s * NSUMS, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (double v : h)
+    if (!std::isfinite(v)) return fail(XPFC_ERR_NONFINITE, "non-finite free-energy sum");
+  const XInst& in = c->inst[s];
+  const double sf = fe_sum(c, in, h[0], h[3], h[4], h[5]);
+  *F = ff_factor(in.p) * sf;
+  *fm = sf / (double)c->N;
+  *mean_rs = h[2] / (double)c->N;
+  return XPFC_OK;
+}
+
+// ---------------------------------------------------------------------------
+// autotuner: times every variant of each pass on scratch buffers and keeps the
+// fastest; results are cached per (precision, grid, instances) for the
+// process.  The variants only regroup lines into workgroups or change register
+// allocation, so the choice never changes the computed field.
+// ---------------------------------------------------------------------------
+std::mutex g_tune_mu;
+std::map<std::array<int, 5>, std::array<int, 3>> g_tune_cache;
+
+int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps, float* ms) {
+  const long long nl = (long long)c->n0 * c->n1;
+  const Batch bt = bt_of(c, c->d_ids);
+  auto launch = [&]() -> int {
+    if (which == 0)
+      return launch_xstep(c->prec, a, b, c->segX, c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
+    if (which == 1) return launch_strided(c->prec, a, c->segY, c->n1, v, -1, c->tw1, bt, ny, c->stream);
+    return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, v, bt, ny, c->twz,
+                        c->twp, c->pz, false, c->stream);
+  };
+  int r = launch();  // warm-up; also tells whether the variant exists
+  if (r == -3) return -3;
+  if (r < 0) return fail(XPFC_ERR_DEVICE, "autotune launch failed (%d)", r);
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  for (int i = 0; i < reps; ++i)
+    if (launch() < 0) return fail(XPFC_ERR_DEVICE, "autotune launch failed");
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  *ms /= (float)reps;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int autotune(xpfc_ctx* c) {
+  // tune with up to 8 instances side by side (the batched launch shape)
+  const int ny = std::min(c->K, 8);
+  const std::array<int, 5> key{c->prec, c->n0, c->n1, c->n2, ny};
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      c->tc_x = it->second[0];
+      c->tc_y = it->second[1];
+      c->ntz = it->second[2];
+      c->tuned = true;
+      drop_graphs(c);
+      return XPFC_OK;
+    }
+  }
+  const size_t sbytes = (size_t)ny * c->stride * c->csize;
+  void *a = nullptr, *b = nullptr;
+  if (hipMalloc(&a, sbytes) != hipSuccess || hipMalloc(&b, sbytes) != hipSuccess) {
+    if (a) (void)hipFree(a);
+    return fail(XPFC_ERR_NOMEM, "autotune scratch allocation failed");
+  }
+  int rc = XPFC_OK;
+  if (hipMemsetAsync(a, 0, sbytes, c->stream) != hipSuccess ||
+      hipMemsetAsync(b, 0, sbytes, c->stream) != hipSuccess)
+    rc = fail(XPFC_ERR_DEVICE, "autotune scratch clear failed");
+  const int reps = c->N * ny >= (1LL << 24) ? 4 : 16;
+  int best[3] = {c->tc_x, c->tc_y, c->ntz};
+  // axis-0: tile width (+XOCC_HINT); axis-1: tile width; real space: block (+ZFREE_REGS)
+  const std::vector<int> xs{4, 8, 16, 4 | XOCC_HINT, 8 | XOCC_HINT, 16 | XOCC_HINT};
+  const std::vector<int> ys{4, 8, 16};
+  const std::vector<int> zs{64, 128, 256, 512, 64 | ZFREE_REGS, 128 | ZFREE_REGS,
+                            256 | ZFREE_REGS, 512 | ZFREE_REGS};
+  for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
+    if (which == 1 && c->n1 <= 1) continue;
+    float bt = 1e30f;
+    for (int v : (which == 0 ? xs : which == 1 ? ys : zs)) {
+      float ms = 0.f;
+      const int r = time_pass(c, which, v, a, b, ny, reps, &ms);
+      if (r == -3) continue;
+      if (r) {
+        rc = r;
+        break;
+      }
+      if (ms < bt) {
+        bt = ms;
+        best[which] = v;
+      }
+    }
+  }
+  (void)hipStreamSynchronize(c->stream);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  if (rc) return rc;
+  c->tc_x = best[0];
+  c->tc_y = best[1];
+  c->ntz = best[2];
+  c->tuned = true;
+  drop_graphs(c);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune_cache[key] = {best[0], best[1], best[2]};
+  return XPFC_OK;
+}
+
+bool autotune_enabled() {
+  const char* e = std::getenv("XPFC_AUTOTUNE");
+  return !(e && e[0] == '0');
+}
+
+}  // namespace
+
+extern "C" int xpfc_autotune(xpfc_ctx* c, int* variants) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  int rc = autotune(c);
+  if (rc) return rc;
+  if (variants) {
+    variants[0] = c->tc_x;
+    variants[1] = c->tc_y;
+    variants[2] = c->ntz;
+  }
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
+  if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  const long long nl = (long long)c->n0 * c->n1;
+  if (v[0] > 0 && strided_blocks(c->prec, c->n0, v[0], c->segX) < 0)
+    return fail(XPFC_ERR_ARG, "tile width %d not available for n0 = %d", v[0], c->n0);
+  if (c->n1 > 1 && v[1] > 0 && strided_blocks(c->prec, c->n1, v[1], c->segY) < 0)
+    return fail(XPFC_ERR_ARG, "tile width %d not available for n1 = %d", v[1], c->n1);
+  if (v[2] > 0 && zpass_blocks(c->prec, c->H, v[2], nl) < 0)
+    return fail(XPFC_ERR_ARG, "real-space block %d not available for n2 = %d", v[2], c->n2);
+  if (v[0] > 0) c->tc_x = v[0];
+  if (c->n1 > 1 && v[1] > 0) c->tc_y = v[1];
+  if (v[2] > 0) c->ntz = v[2];
+  c->tuned = true;  // explicit choice: the autotuner leaves it alone
+  drop_graphs(c);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_get_variants(const xpfc_ctx* c, int* v) {
+  if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  v[0] = c->tc_x;
+  v[1] = c->tc_y;
+  v[2] = c->ntz;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_set_graphs(xpfc_ctx* c, int on) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  c->use_graphs = on != 0;
+  if (!c->use_graphs) drop_graphs(c);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (nsteps < 0) return fail(XPFC_ERR_ARG, "nsteps < 0");
+  if (c->active.empty() || nsteps == 0) return XPFC_OK;
+  for (int s : c->active)
+    if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "instance %d: no field set", s);
+  if (!c->tuned && autotune_enabled()) {
+    int rc = autotune(c);
+    if (rc) return rc;
+  }
+  for (int s : c->active) {
+    XInst& in = c->inst[s];
+    if (in.step == 0 && in.p.t_sample > 0 && in.samples.empty()) {
+      // main.cu:286: FE of the initial field
+      double F, fm, mrs;
+      int rc = fe_now(c, s, &F, &fm, &mrs);
+      if (rc) return rc;
+      xpfc_sample r{};
+      r.step = 0;
+      r.has_fe = 1;
+      r.F_ref = F;
+      r.f_mean = fm;
+      r.mean = mrs;
+      r.rmse = NAN;
+      in.samples.push_back(r);
+    }
+  }
+  auto any_samples = [&](long long off) {
+    for (int s : c->active)
+      if (inst_samples(c->inst[s], c->inst[s].step + off)) return true;
+    return false;
+  };
+  long long left = nsteps;
+  while (left > 0) {
+    if (any_samples(0)) {
+      if ((int)c->pending.size() == SUM_ROWS) {
+        int rc = drain(c);
+        if (rc) return rc;
+      }
+      XPending pr;
+      pr.row = (int)c->pending.size();
+      for (int s : c->active)
+        if (inst_samples(c->inst[s], c->inst[s].step)) pr.who.emplace_back(s, c->inst[s].step);
+      int rc = launch_one_step(c, true, pr.row);
+      if (rc) return rc;
+      c->pending.push_back(pr);
+      advance(c, 1);
+      --left;
+      continue;
+    }
+    // a run of unsampled steps up to the next sampled one
+    long long run = 1;
+    while (run < left && !any_samples(run)) ++run;
+    int rc = run_plain_steps(c, run);
+    if (rc) return rc;
+    left -= run;
+  }
+  return drain(c);
+}
+
+extern "C" long long xpfc_inst_current_step(const xpfc_ctx* c, int s) {
+  return (c && s >= 0 && s < c->K) ? c->inst[s].step : 0;
+}
+extern "C" long long xpfc_current_step(const xpfc_ctx* c) { return xpfc_inst_current_step(c, 0); }
+
+extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
+  if (!c || !ms) return fail(XPFC_ERR_ARG, "null argument");
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c->stream));
+  int rc = xpfc_step(c, nsteps);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipEventSynchronize(e1));
+  HIPCHK(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return rc;
+}
+
+extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
+  if (!c || !ms || nsteps <= 0) return fail(XPFC_ERR_ARG, "bad arguments");
+  for (int s : c->active)
+    if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
+  if (!c->tuned && autotune_enabled()) {
+    int rc = autotune(c);
+    if (rc) return rc;
+  }
+  std::vector<hipEvent_t> ev((size_t)nsteps * 4 + 1);
+  for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+  HIPCHK(hipEventRecord(ev[0], c->stream));
+  for (int i = 0; i < nsteps; ++i) {
+    int rc = launch_one_step(c, false, 0, &ev[(size_t)i * 4 + 1]);
+    if (rc) return rc;
+    advance(c, 1);
+  }
+  HIPCHK(hipEventSynchronize(ev.back()));
+  double acc[4] = {0, 0, 0, 0};
+  for (int i = 0; i < nsteps; ++i)
+    for (int k = 0; k < 4; ++k) {
+      float t = 0.f;
+      HIPCHK(hipEventElapsedTime(&t, ev[(size_t)i * 4 + k], ev[(size_t)i * 4 + k + 1]));
+      acc[k] += t;
+    }
+  float tot = 0.f;
+  HIPCHK(hipEventElapsedTime(&tot, ev[0], ev.back()));
+  for (int k = 0; k < 4; ++k) ms[k] = acc[k] / nsteps;
+  ms[4] = (double)tot / nsteps;
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_sync(xpfc_ctx* c) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_inst_num_samples(const xpfc_ctx* c, int s) {
+  return (c && s >= 0 && s < c->K) ? (int)c->inst[s].samples.size() : 0;
+}
+extern "C" int xpfc_inst_get_samples(const xpfc_ctx* c, int s, xpfc_sample* out, int max) {
+  if (!c || s < 0 || s >= c->K || (max > 0 && !out)) return fail(XPFC_ERR_ARG, "bad arguments");
+  const auto& v = c->inst[s].samples;
+  const int n = std::min(max, (int)v.size());
+  for (int i = 0; i < n; ++i) out[i] = v[i];
+  return n;
+}
+extern "C" void xpfc_inst_clear_samples(xpfc_ctx* c, int s) {
+  if (c && s >= 0 && s < c->K) c->inst[s].samples.clear();
+}
+extern "C" int xpfc_num_samples(const xpfc_ctx* c) { return xpfc_inst_num_samples(c, 0); }
+extern "C" int xpfc_get_samples(const xpfc_ctx* c, xpfc_sample* out, int max) {
+  return xpfc_inst_get_samples(c, 0, out, max);
+}
+extern "C" void xpfc_clear_samples(xpfc_ctx* c) { xpfc_inst_clear_samples(c, 0); }
+
+// ---------------------------------------------------------------------------
+// diagnostics
+// ---------------------------------------------------------------------------
+extern "C" int xpfc_inst_fe_calc(xpfc_ctx* c, int s, double* F_ref, double* f_mean) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!F_ref || !f_mean) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
+  double m;
+  return fe_now(c, s, F_ref, f_mean, &m);
+}
+
+extern "C" int xpfc_inst_avg_calc(xpfc_ctx* c, int s, double* mean) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!mean) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
+  *mean = c->inst[s].mean;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_inst_sse_calc(xpfc_ctx* c, int s, double* rmse) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!rmse) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->inst[s].have_rmse) return fail(XPFC_ERR_STATE, "no sampled step yet");
+  *rmse = c->inst[s].last_rmse;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_fe_calc(xpfc_ctx* c, double* F_ref, double* f_mean) {
+  return xpfc_inst_fe_calc(c, 0, F_ref, f_mean);
+}
+extern "C" int xpfc_avg_calc(xpfc_ctx* c, double* mean) { return xpfc_inst_avg_calc(c, 0, mean); }
+extern "C" int xpfc_sse_calc(xpfc_ctx* c, double* rmse) { return xpfc_inst_sse_calc(c, 0, rmse); }
+
+// ---------------------------------------------------------------------------
+// operator level (device pointers; instance 0's physics)
+// ---------------------------------------------------------------------------
+extern "C" int xpfc_malloc(xpfc_ctx* c, void** d, size_t bytes) {
+  if (!c || !d) return fail(XPFC_ERR_ARG, "null argument");
+  if (hipMalloc(d, bytes) != hipSuccess) return fail(XPFC_ERR_NOMEM, "hipMalloc(%zu) failed", bytes);
+  return XPFC_OK;
+}
+extern "C" int xpfc_free(xpfc_ctx* c, void* d) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipFree(d));
+  return XPFC_OK;
+}
+extern "C" int xpfc_memcpy_h2d(xpfc_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+extern "C" int xpfc_memcpy_d2h(xpfc_ctx* c, void* dst, const void* src, size_t bytes) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+
+namespace {
+int op_done(xpfc_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+const Phys& ph0(const xpfc_ctx* c) { return c->inst[0].ia.ph; }
+}  // namespace
+
+extern "C" int xpfc_power_functions(xpfc_ctx* c, int p, const void* src, void* dst) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!src || !dst || p < 0) return fail(XPFC_ERR_ARG, "bad power_functions arguments");
+  LCHK(launch_power(c->prec, src, dst, c->N, p, c->stream));
+  return op_done(c);
+}
+
+extern "C" int xpfc_convolve_fields(xpfc_ctx* c, const void* psi_k, void* out) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!psi_k || !out) return fail(XPFC_ERR_ARG, "null buffer");
+  LCHK(launch_convolve(c->prec, psi_k, out, c->g, ph0(c), c->n2, c->stream));
+  return op_done(c);
+}
+
+extern "C" int xpfc_euler_step(xpfc_ctx* c, const void* psi_k, const void* p2_k, const void* p3_k,
+                               void* out) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!psi_k || !p2_k || !p3_k || !out) return fail(XPFC_ERR_ARG, "null buffer");
+  const xpfc_params& p = c->inst[0].p;
+  LCHK(launch_euler(c->prec, psi_k, p2_k, p3_k, out, c->g, ph0(c), p.a, p.b, c->n2, c->stream));
+  return op_done(c);
+}
+
+extern "C" int xpfc_residual_calc(xpfc_ctx* c, const void* kp1, const void* k0, const void* p2,
+                                  const void* p3, void* out) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!kp1 || !k0 || !p2 || !p3 || !out) return fail(XPFC_ERR_ARG, "null buffer");
+  const xpfc_params& p = c->inst[0].p;
+  LCHK(launch_residual(c->prec, kp1, k0, p2, p3, out, c->g, ph0(c), p.a, p.b, c->n2, c->stream));
+  return op_done(c);
+}
+
+extern "C" int xpfc_scaling_step(xpfc_ctx* c, void* buf, int spectral) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!buf) return fail(XPFC_ERR_ARG, "null buffer");
+  const long long n = spectral ? 2LL * c->n0 * c->n1 * (c->H + 1) : c->N;
+  LCHK(launch_scale(c->prec, buf, n, 1.0 / (double)c->N, c->stream));
+  return op_done(c);
+}
+
+namespace {
+// scratch table entry K: custom real-space constants for operator calls
+int scratch_args(xpfc_ctx* c, double mean, double scale) {
+  InstArgs ia = c->inst[0].ia;
+  ia.za.mean = mean;
+  ia.za.scale = scale;
+  HIPCHK(hipMemcpyAsync(c->d_ia + c->K, &ia, sizeof(ia), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+}  // namespace
+
+extern "C" int xpfc_fft_r2c(xpfc_ctx* c, const void* real_in, void* spec_out) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!real_in || !spec_out) return fail(XPFC_ERR_ARG, "null buffer");
+  const long long nl = (long long)c->n0 * c->n1;
+  int rc = scratch_args(c, 0.0, 1.0);  // unnormalised, as cuFFT
+  if (rc) return rc;
+  const Batch bs{c->d_ids + c->K, c->d_ia, 0, 0, 0};  // no instance offsets
+  LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, real_in, nullptr, c->aux, ny_of(c, c->aux), nl, c->H, 256, bs, 1,
+                    c->twz, c->twp, nullptr, false, c->stream));
+  if ((rc = forward_xy(c, c->aux, bt_one(c, 0), 1, true))) return rc;
+  LCHK(launch_unpack(c->prec, c->aux, spec_out, (int)nl, c->H, c->stream));
+  return op_done(c);
+}
+
+extern "C" int xpfc_fft_c2r(xpfc_ctx* c, const void* spec_in, void* real_out) {
+  if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!spec_in || !real_out) return fail(XPFC_ERR_ARG, "null buffer");
+  const long long nl = (long long)c->n0 * c->n1;
+  int rc = scratch_args(c, 0.0, 1.0);  // unnormalised, as cuFFT
+  if (rc) return rc;
+  LCHK(launch_pack(c->prec, spec_in, c->aux, (int)nl, c->H, c->stream));
+  if ((rc = inverse_xy(c, c->aux, bt_one(c, 0), 1))) return rc;
+  const Batch bs{c->d_ids + c->K, c->d_ia, 0, 0, 0};
+  LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), real_out, nullptr, nl, c->H, 256, bs, 1,
+                    c->twz, c->twp, nullptr, false, c->stream));
+  return op_done(c);
+}
+
+// ---------------------------------------------------------------------------
+// VTK output (write_vtk.cu:4-35)
+// ---------------------------------------------------------------------------
+extern "C" int xpfc_inst_write_vtk(xpfc_ctx* c, int s, const char* path) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!path) return fail(XPFC_ERR_ARG, "null path");
+  std::vector<double> psi((size_t)c->N);
+  if ((rc = xpfc_inst_get_field(c, s, psi.data()))) return rc;
+  std::ofstream f(path);
+  if (!f) return fail(XPFC_ERR_IO, "cannot open %s for writing", path);
+  const xpfc_params& p = c->inst[s].p;
+  const int nx = p.n[0], ny = p.n[1], nz = p.dim == 3 ? p.n[2] : 1;
+  // write_vtk.cu:15-27 (default ostream formatting, x fastest)
+  f << "# vtk DataFile Version 3.0\n";
+  f << "Density Data\n";
+  f << "ASCII\n";
+  f << "DATASET STRUCTURED_POINTS\n";
+  f << "DIMENSIONS " << nx << " " << ny << " " << nz << "\n";
+  f << "ORIGIN 0 0 0\n";
+  f << "SPACING " << double(p.L[0]) / double(nx) << " " << double(p.L[1]) / double(ny) << " "
+    << (p.dim == 3 ? double(p.L[2]) / double(nz) : 1.0) << "\n";
+  f << "POINT_DATA " << (long long)nx * ny * nz << "\n";
+  f << "SCALARS DensityData float\n";
+  f << "LOOKUP_TABLE default\n";
+  for (int l = 0; l < nz; ++l)
+    for (int j = 0; j < ny; ++j)
+      for (int i = 0; i < nx; ++i) f << psi[((size_t)i * ny + j) * nz + l] << "\n";
+  if (!f) return fail(XPFC_ERR_IO, "write to %s failed", path);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_write_vtk(xpfc_ctx* c, const char* path) { return xpfc_inst_write_vtk(c, 0, path); }

@@ -1,0 +1,94 @@
+// Private header of the host runtime (context layout, error helpers).
+#pragma once
+#include "../../include/xpfc.h"
+#include "xpfc_internal.hpp"
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <utility>
+#include <vector>
+
+namespace xpfc {
+
+// Records a message for xpfc_last_error() and returns `code`.
+int fail(int code, const char* fmt, ...);
+
+// Reference initial-condition helpers shared with the context code.
+std::vector<double> linspace_ref(double x1, double x2, int N);
+void fix_mean(std::vector<double>& psi, double psi0);
+int read_vtk_file(const char* path, int* nx, int* ny, double* spacing, std::vector<double>* vals);
+double c2hat_host(double kmag, const xpfc_params& p);
+
+}  // namespace xpfc
+
+#define HIPCHK(expr)                                                                        \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess)                                                                   \
+      return xpfc::fail(XPFC_ERR_DEVICE, "%s failed: %s", #expr, hipGetErrorString(e_));   \
+  } while (0)
+
+#define LCHK(expr)                                                                          \
+  do {                                                                                      \
+    int r_ = (expr);                                                                        \
+    if (r_ < 0) {                                                                           \
+      hipError_t e_ = hipGetLastError();                                                    \
+      return xpfc::fail(XPFC_ERR_DEVICE, "%s failed (%d): %s", #expr, r_,                   \
+                        hipGetErrorString(e_));                                             \
+    }                                                                                       \
+  } while (0)
+
+// Per-instance state of a context.
+struct XInst {
+  xpfc_params p{};       // physics of this instance (grid equal to the context's)
+  xpfc::InstArgs ia{};   // host copy of the device table entry
+  double mean = 0.0;     // FP64 mean density (the k = 0 mode)
+  double c2zero = 0.0;   // c2(k = 0), for the Parseval k = 0 term
+  long long step = 0;
+  bool have_field = false;
+  std::vector<xpfc_sample> samples;
+  double last_rmse = 0.0;
+  bool have_rmse = false;
+};
+
+// One sampled step awaiting read-back: its row in the sums buffer and the
+// instances that sample at it (with their step numbers).
+struct XPending {
+  int row;
+  std::vector<std::pair<int, long long>> who;
+};
+
+struct xpfc_ctx {
+  xpfc_params p{};  // instance 0's parameters: the grid of every instance
+  int prec = 0;
+  int n0 = 0, n1 = 0, n2 = 0, H = 0;
+  long long N = 0, nmain = 0, nny = 0;
+  size_t csize = 0, rsize = 0;
+  int K = 1;                 // instances
+  long long stride = 0;      // complex elements per instance spectrum (nmain + nny)
+  void* psi = nullptr;       // [K] packed spectra of psi - mean (state)
+  void* work = nullptr;      // [K] nonlinear term / inverse-transform work fields
+  void* aux = nullptr;       // [K] diagnostics / field I/O scratch
+  void* tw0 = nullptr;       // exp(-2 pi i m / n0)
+  void* tw1 = nullptr;       // exp(-2 pi i m / n1)
+  void* twz = nullptr;       // exp(-2 pi i m / H)
+  void* twp = nullptr;       // exp(-2 pi i k / n2), k < H
+  double* px = nullptr;      // [K][npx_cap][XPART]
+  double* pz = nullptr;      // [K][npz_cap][ZPART]
+  double* sums = nullptr;    // [SUM_ROWS][K][NSUMS]
+  xpfc::InstArgs* d_ia = nullptr;  // [K + 1]; entry K: scratch for operator calls
+  int* d_ids = nullptr;      // [K + 1] = 0 .. K
+  int* d_active = nullptr;   // [K] active instance list
+  int npx_cap = 0, npz_cap = 0;
+  hipStream_t stream = nullptr;
+  xpfc::KGeom g{};
+  xpfc::Segs segX{}, segY{};
+  int tc_x = 0, tc_y = 0, ntz = 256;  // pass variants
+  bool tuned = false;
+  bool use_graphs = true;
+  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  std::vector<XInst> inst;
+  std::vector<int> active;
+  std::vector<XPending> pending;
+};

@@ -51,6 +51,25 @@ struct ZArgs {
 };
 
 enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
+
+// Per-instance constants of a context holding several simulations of one grid
+// (device array indexed by instance slot).
+struct InstArgs {
+  Phys ph;
+  ZArgs za;
+};
+
+// Which instances a launch covers: blockIdx.y (blockIdx.x for the finalize)
+// indexes `slots`; instance `s` has its spectra `s * stride` complex elements
+// from the base pointers, its X partials `s * pstride` and its real-space
+// partials `s * pzstride` doubles from the partial base pointers.
+struct Batch {
+  const int* slots;
+  const InstArgs* ia;
+  long long stride;
+  long long pstride;
+  long long pzstride;
+};
 // Register-allocation variant flags (the arithmetic is identical):
 // or-ed into the real-space workgroup size: no occupancy hint;
 // or-ed into the axis-0 tile width: budget for two 1024-thread workgroups/CU.
@@ -68,18 +87,22 @@ constexpr int NSUMS = 6;
 // workgroup size (64..512) of the real-space pass; a combination a size does
 // not support returns -3.  launch_xstep returns its workgroup count (= number
 // of X partials) on success.
+// `ny` is the number of instances covered (entries of bt.slots used).
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
-                   hipStream_t s);
+                   const Batch& bt, int ny, hipStream_t s);
 int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
-                 const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s);
+                 const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
+                 hipStream_t s);
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                 long long nlines, int H, int ntz, const ZArgs& za, const void* tw, const void* twp,
-                 double* partial, bool sample, hipStream_t s);
+                 long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
+                 const void* twp, double* partial, bool sample, hipStream_t s);
 int zpass_blocks(int prec, int H, int ntz, long long nlines);
 int strided_blocks(int prec, int L, int tc, const Segs& segs);
-int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s);
+// out[slot * NSUMS + k]: the six sums of every listed instance
+int launch_finalize(const double* px, int nx, const double* pz, int nz, const Batch& bt, int ny,
+                    double* out, hipStream_t s);
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
-                       const Phys& ph, double* partial, int* nblocks, hipStream_t s);
+                       const Batch& bt, double* partial, int* nblocks, hipStream_t s);
 
 // operator-level kernels on natural layouts (real [N], spectrum [n0][n1][n2/2+1])
 int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s);

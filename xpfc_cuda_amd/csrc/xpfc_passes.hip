@@ -70,10 +70,11 @@ __device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, in
 // ---------------------------------------------------------------------------
 template <typename T, int L, int TC, int DIR>
 __global__ __launch_bounds__((tile_threads<L, TC>()))
-void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw) {
+void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   __shared__ C lds[L * (TC + PADT)];
+  buf += (long long)bt.slots[blockIdx.y] * bt.stride;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
   const int rs = loc.sg->rs;
@@ -96,11 +97,16 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
 template <typename T, int L, int TC, bool SAMPLE, bool OCC>
 __global__ __launch_bounds__((tile_threads<L, TC>()), (OCC ? (sizeof(T) == 4 ? 8 : 4) : 1))
 void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
-             const cpx<T>* __restrict__ tw, KGeom g, Phys ph, double* __restrict__ partial) {
+             const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   constexpr int NT = tile_threads<L, TC>();
   __shared__ C lds[L * (TC + PADT)];
+  const int slot = bt.slots[blockIdx.y];
+  work += (long long)slot * bt.stride;
+  psi += (long long)slot * bt.stride;
+  partial += (long long)slot * bt.pstride;
+  const Phys& ph = bt.ia[slot].ph;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
   const int rs = loc.sg->rs;
@@ -203,10 +209,20 @@ template <int H, int E> struct ZIdx {
 template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE>
 __global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
 void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
-             cpx<T>* __restrict__ out_ny, long long nlines, ZArgs za,
+             cpx<T>* __restrict__ out_ny, long long nlines, Batch bt,
              const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
              double* __restrict__ partial) {
   using C = cpx<T>;
+  const int slot = bt.slots[blockIdx.y];
+  {
+    const long long so = (long long)slot * bt.stride;  // complex elements
+    in = reinterpret_cast<const C*>(in) + so;
+    out = reinterpret_cast<C*>(out) + so;
+    if (in_ny) in_ny += so;
+    if (out_ny) out_ny += so;
+    partial += (long long)slot * bt.pzstride;
+  }
+  const ZArgs za = bt.ia[slot].za;
   constexpr int E = z_elems<T, H>();
   constexpr int TPL = z_tpl<T, H>();
   constexpr int G = NTZ / TPL;
@@ -316,9 +332,14 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
 // final fixed-order reduction of the per-workgroup partials
 // out = {sum c2|psi_k|^2, sum |R_k|^2, sum psi, sum psi^2, sum psi^3, sum psi^4}
 // ---------------------------------------------------------------------------
+// one workgroup per listed instance: its partials -> out[slot * NSUMS ...]
 __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ px, int nx,
                                                    const double* __restrict__ pz, int nz,
-                                                   double* __restrict__ out) {
+                                                   Batch bt, double* __restrict__ out) {
+  const int slot = bt.slots[blockIdx.x];
+  px += (long long)slot * bt.pstride;
+  pz += (long long)slot * bt.pzstride;
+  out += (long long)slot * NSUMS;
   double a[NSUMS] = {0, 0, 0, 0, 0, 0};
   for (int i = threadIdx.x; i < nx; i += 1024) {
     a[0] += px[(long long)i * XPART + 0];
@@ -334,8 +355,12 @@ __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ px
 // k-space Parseval sum  sum_k w c2(k) |psi_k|^2  over the packed spectrum
 template <typename T>
 __global__ __launch_bounds__(256) void k_kspace_sums(const cpx<T>* __restrict__ psi, long long nmain,
-                                                     long long nny, KGeom g, Phys ph,
+                                                     long long nny, KGeom g, Batch bt,
                                                      double* __restrict__ partial) {
+  const int slot = bt.slots[0];
+  psi += (long long)slot * bt.stride;
+  partial += (long long)slot * bt.pstride;
+  const Phys& ph = bt.ia[slot].ph;
   double acc[XPART] = {0.0, 0.0};
   const long long tot = nmain + nny;
   for (long long e = (long long)blockIdx.x * 256 + threadIdx.x; e < tot;
@@ -411,18 +436,18 @@ static Segs fill_tiles(const Segs& in, int TC) {
 
 template <typename T>
 static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const void* tw,
-                     hipStream_t s) {
+                     const Batch& bt, int ny, hipStream_t s) {
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       const int nb = sg.tiles0 + sg.tiles1;
-      if (nb == 0) return 0;
+      if (nb == 0 || ny == 0) return 0;
       if (dir < 0)
-        hipLaunchKernelGGL((k_strided<T, LL, TT, -1>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
-                           (cpx<T>*)buf, sg, (const cpx<T>*)tw);
+        hipLaunchKernelGGL((k_strided<T, LL, TT, -1>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
+                           s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
       else
-        hipLaunchKernelGGL((k_strided<T, LL, TT, +1>), dim3(nb), dim3(tile_threads<LL, TT>()), 0, s,
-                           (cpx<T>*)buf, sg, (const cpx<T>*)tw);
+        hipLaunchKernelGGL((k_strided<T, LL, TT, +1>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
+                           s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
     } else {
       return -3;
     }
@@ -431,9 +456,9 @@ static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const
 }
 
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
-                   hipStream_t s) {
-  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, s)
-                   : strided_T<double>(buf, segs, L, tc, dir, tw, s);
+                   const Batch& bt, int ny, hipStream_t s) {
+  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, bt, ny, s)
+                   : strided_T<double>(buf, segs, L, tc, dir, tw, bt, ny, s);
 }
 
 template <typename T>
@@ -458,7 +483,8 @@ int strided_blocks(int prec, int L, int tc, const Segs& segs) {
 // (sampled steps always use the unconstrained build)
 template <typename T>
 static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, const void* tw,
-                   const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s) {
+                   const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
+                   hipStream_t s) {
   int nb = 0;
   const bool occ = (tc & XOCC_HINT) != 0;
   tc &= ~XOCC_HINT;
@@ -466,15 +492,17 @@ static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, cons
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       nb = sg.tiles0 + sg.tiles1;
+      if (ny == 0) return nb;
+      const dim3 grid(nb, ny), block(tile_threads<LL, TT>());
       if (sample)
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, true, false>), dim3(nb), dim3(tile_threads<LL, TT>()),
-                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, true, false>), grid, block, 0, s, (cpx<T>*)work,
+                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
       else if (occ)
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, true>), dim3(nb), dim3(tile_threads<LL, TT>()),
-                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, true>), grid, block, 0, s, (cpx<T>*)work,
+                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
       else
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, false>), dim3(nb), dim3(tile_threads<LL, TT>()),
-                           0, s, (cpx<T>*)work, (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, ph, partial);
+        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, false>), grid, block, 0, s, (cpx<T>*)work,
+                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
     } else {
       return -3;
     }
@@ -483,9 +511,10 @@ static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, cons
 }
 
 int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
-                 const KGeom& g, const Phys& ph, double* partial, bool sample, hipStream_t s) {
-  return prec == 1 ? xstep_T<float>(work, psi, segs, L, tc, tw, g, ph, partial, sample, s)
-                   : xstep_T<double>(work, psi, segs, L, tc, tw, g, ph, partial, sample, s);
+                 const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
+                 hipStream_t s) {
+  return prec == 1 ? xstep_T<float>(work, psi, segs, L, tc, tw, g, bt, ny, partial, sample, s)
+                   : xstep_T<double>(work, psi, segs, L, tc, tw, g, bt, ny, partial, sample, s);
 }
 
 template <typename T>
@@ -503,36 +532,37 @@ int zpass_blocks(int prec, int H, int ntz, long long nlines) {
   return prec == 1 ? zblocks_T<float>(H, ntz, nlines) : zblocks_T<double>(H, ntz, nlines);
 }
 
-template <typename T, int H, int NZ, bool MIR, int MODE>
+template <typename T, int H, int NZ, bool FRE, int MODE>
 static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, long long nlines,
-                    const ZArgs& za, const void* tw, const void* twp, double* partial, bool sample,
-                    hipStream_t s) {
+                    const Batch& bt, int ny, const void* tw, const void* twp, double* partial,
+                    bool sample, hipStream_t s) {
   constexpr int G = NZ / z_tpl<T, H>();
-  const int nb = (int)((nlines + G - 1) / G);
+  const dim3 grid((unsigned)((nlines + G - 1) / G), ny);
   if (sample)
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, MIR, MODE, true>), dim3(nb), dim3(NZ), 0, s, in,
-                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, true>), grid, dim3(NZ), 0, s, in,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
   else
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, MIR, MODE, false>), dim3(nb), dim3(NZ), 0, s, in,
-                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, za,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, false>), grid, dim3(NZ), 0, s, in,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
 }
 
 // ntz: workgroup size (64..512); +ZFREE_REGS drops the occupancy hint
 template <typename T>
 static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                   long long nlines, int H, int ntz, const ZArgs& za, const void* tw,
+                   long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
                    const void* twp, double* partial, bool sample, hipStream_t s) {
+  if (ny == 0) return 0;
   const bool fre = (ntz & ZFREE_REGS) != 0;
   ntz &= ~ZFREE_REGS;
   if (mode == Z_STEP) {
     XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
       if constexpr (z_ok<T, LL, NZ>()) {
         if (fre)
-          zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
+          zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s);
         else
-          zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s);
+          zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s);
       } else {
         return -3;
       }
@@ -542,9 +572,9 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
     XPFC_LSWITCH(H, {
       constexpr int NZ = 256;
       switch (mode) {
-        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
-        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, false, s); break;
-        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, za, tw, twp, partial, sample, s); break;
+        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, false, s); break;
+        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, false, s); break;
+        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s); break;
         default: return -1;
       }
     });
@@ -553,30 +583,32 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
 }
 
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                 long long nlines, int H, int ntz, const ZArgs& za, const void* tw, const void* twp,
-                 double* partial, bool sample, hipStream_t s) {
+                 long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
+                 const void* twp, double* partial, bool sample, hipStream_t s) {
   return prec == 1
-             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, ntz, za, tw, twp, partial, sample, s)
-             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, ntz, za, tw, twp, partial, sample, s);
+             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, ntz, bt, ny, tw, twp, partial, sample, s)
+             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, ntz, bt, ny, tw, twp, partial, sample, s);
 }
 
-int launch_finalize(const double* px, int nx, const double* pz, int nz, double* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, px, nx, pz, nz, out);
+int launch_finalize(const double* px, int nx, const double* pz, int nz, const Batch& bt, int ny,
+                    double* out, hipStream_t s) {
+  if (ny == 0) return 0;
+  hipLaunchKernelGGL(k_finalize, dim3(ny), dim3(1024), 0, s, px, nx, pz, nz, bt, out);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
-                       const Phys& ph, double* partial, int* nblocks, hipStream_t s) {
+                       const Batch& bt, double* partial, int* nblocks, hipStream_t s) {
   long long nb = (nmain + nny + 255) / 256;
   if (nb > 4096) nb = 4096;
   if (nb < 1) nb = 1;
   *nblocks = (int)nb;
   if (prec == 1)
     hipLaunchKernelGGL(k_kspace_sums<float>, dim3(nb), dim3(256), 0, s, (const cpx<float>*)psi,
-                       nmain, nny, g, ph, partial);
+                       nmain, nny, g, bt, partial);
   else
     hipLaunchKernelGGL(k_kspace_sums<double>, dim3(nb), dim3(256), 0, s, (const cpx<double>*)psi,
-                       nmain, nny, g, ph, partial);
+                       nmain, nny, g, bt, partial);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
