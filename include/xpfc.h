@@ -124,12 +124,16 @@ long long xpfc_num_cells(const xpfc_ctx* ctx);
 /* ---- the time-stepping hot path (main.cu:306-435) ------------------------ */
 /* Advance nsteps steps.  Every t_sample steps the FE / RMSE records of the
  * reference schedule (SSE when n % t_sample == 0, FE when (n+1) % t_sample
- * == 0) are appended to the sample log. */
+ * == 0) are appended to the sample log.  The steps are queued on the context
+ * stream and the call returns; the sampled sums are read back by xpfc_sync and
+ * by the calls that read or change the sample log or an instance's state, and
+ * a non-finite sum is reported there (XPFC_ERR_NONFINITE). */
 int xpfc_step(xpfc_ctx* ctx, int nsteps);
 long long xpfc_current_step(const xpfc_ctx* ctx);
-/* Sample log: number of records, copy up to max of them, clear. */
-int xpfc_num_samples(const xpfc_ctx* ctx);
-int xpfc_get_samples(const xpfc_ctx* ctx, xpfc_sample* out, int max);
+/* Sample log: number of records, copy up to max of them (both return minus a
+ * status code on failure), clear. */
+int xpfc_num_samples(xpfc_ctx* ctx);
+int xpfc_get_samples(xpfc_ctx* ctx, xpfc_sample* out, int max);
 void xpfc_clear_samples(xpfc_ctx* ctx);
 /* Time nsteps steps with hipEvents on the context stream (helpers_header.cuh:12-48). */
 int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
@@ -154,7 +158,7 @@ int xpfc_get_variants(const xpfc_ctx* ctx, int* v);
  * 50 steps and replayed (on by default; XPFC_GRAPHS=0 or on = 0 launches every
  * kernel eagerly).  Both give identical fields. */
 int xpfc_set_graphs(xpfc_ctx* ctx, int on);
-/* Block until all work of the context has finished. */
+/* Block until all work of the context has finished; read back pending samples. */
 int xpfc_sync(xpfc_ctx* ctx);
 
 /* ---- diagnostics ----------------------------------------------------------- */
@@ -220,8 +224,8 @@ int xpfc_inst_initialize_noise(xpfc_ctx* ctx, int s, double amplitude, unsigned 
 int xpfc_inst_set_field(xpfc_ctx* ctx, int s, const double* host_real);
 int xpfc_inst_get_field(xpfc_ctx* ctx, int s, double* host_real);
 long long xpfc_inst_current_step(const xpfc_ctx* ctx, int s);
-int xpfc_inst_num_samples(const xpfc_ctx* ctx, int s);
-int xpfc_inst_get_samples(const xpfc_ctx* ctx, int s, xpfc_sample* out, int max);
+int xpfc_inst_num_samples(xpfc_ctx* ctx, int s);
+int xpfc_inst_get_samples(xpfc_ctx* ctx, int s, xpfc_sample* out, int max);
 void xpfc_inst_clear_samples(xpfc_ctx* ctx, int s);
 int xpfc_inst_fe_calc(xpfc_ctx* ctx, int s, double* F_ref, double* f_mean);
 int xpfc_inst_avg_calc(xpfc_ctx* ctx, int s, double* mean);

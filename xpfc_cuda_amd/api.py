@@ -410,8 +410,12 @@ class Context:
 
     def samples(self) -> List[Sample]:
         n = lib().xpfc_num_samples(self.handle)
+        if n < 0:
+            _check(-n, "xpfc_num_samples")
         arr = (_Sample * max(n, 1))()
         got = lib().xpfc_get_samples(self.handle, arr, n)
+        if got < 0:
+            _check(-got, "xpfc_get_samples")
         return [Sample(int(s.step), bool(s.has_fe), bool(s.has_sse), s.F_ref, s.f_mean, s.mean,
                        s.rmse) for s in arr[:got]]
 
@@ -583,10 +587,12 @@ class BatchContext:
 
     def samples(self, s: int) -> List[Sample]:
         n = lib().xpfc_inst_num_samples(self.handle, s)
+        if n < 0:
+            _check(-n, "xpfc_inst_num_samples")
         arr = (_Sample * max(n, 1))()
         got = lib().xpfc_inst_get_samples(self.handle, s, arr, n)
         if got < 0:
-            _check(got, "xpfc_inst_get_samples")
+            _check(-got, "xpfc_inst_get_samples")
         return [Sample(int(r.step), bool(r.has_fe), bool(r.has_sse), r.F_ref, r.f_mean, r.mean,
                        r.rmse) for r in arr[:got]]
 

@@ -139,6 +139,10 @@ void* inst_ptr(const xpfc_ctx* c, void* base, int s) {
   return (char*)base + (size_t)s * c->stride * c->csize;
 }
 
+// Reads the sums of pending sampled steps back into the sample logs (defined
+// with the step driver).
+int drain(xpfc_ctx* c);
+
 int check_inst(const xpfc_ctx* c, int s) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (s < 0 || s >= c->K) return fail(XPFC_ERR_ARG, "instance %d out of range [0, %d)", s, c->K);
@@ -299,6 +303,7 @@ extern "C" int xpfc_inst_set_params(xpfc_ctx* c, int s, const xpfc_params* p) {
     q.L[2] = 1.0;
   }
   if (!same_grid(c->p, q)) return fail(XPFC_ERR_ARG, "grid/precision differ from the context's");
+  if ((rc = drain(c)) != XPFC_OK) return rc;  // pending records use the old physics
   if ((rc = build_inst(c, q, &c->inst[s])) != XPFC_OK) return rc;
   return upload_inst(c, s);
 }
@@ -306,6 +311,7 @@ extern "C" int xpfc_inst_set_params(xpfc_ctx* c, int s, const xpfc_params* p) {
 extern "C" int xpfc_inst_reset(xpfc_ctx* c, int s) {
   int rc = check_inst(c, s);
   if (rc) return rc;
+  if ((rc = drain(c)) != XPFC_OK) return rc;
   XInst& in = c->inst[s];
   in.step = 0;
   in.samples.clear();
@@ -395,6 +401,7 @@ extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!host) return fail(XPFC_ERR_ARG, "null field");
+  if ((rc = drain(c)) != XPFC_OK) return rc;  // pending records use the old mean
   // Avg_calc.cu:3-18 mean, FP64, row-major order
   double sum = 0.0;
   for (long long i = 0; i < c->N; ++i) sum += host[i];
@@ -908,7 +915,7 @@ extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
     if (rc) return rc;
     left -= run;
   }
-  return drain(c);
+  return XPFC_OK;  // samples are read back by the next call that needs them
 }
 
 extern "C" long long xpfc_inst_current_step(const xpfc_ctx* c, int s) {
@@ -928,7 +935,8 @@ extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
   HIPCHK(hipEventElapsedTime(ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  return rc;
+  if (rc) return rc;
+  return drain(c);
 }
 
 extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
@@ -966,24 +974,30 @@ extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
 extern "C" int xpfc_sync(xpfc_ctx* c) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
-  return XPFC_OK;
+  return drain(c);
 }
 
-extern "C" int xpfc_inst_num_samples(const xpfc_ctx* c, int s) {
-  return (c && s >= 0 && s < c->K) ? (int)c->inst[s].samples.size() : 0;
+// counts on success, minus the status code on failure
+extern "C" int xpfc_inst_num_samples(xpfc_ctx* c, int s) {
+  if (!c || s < 0 || s >= c->K) return -fail(XPFC_ERR_ARG, "bad arguments");
+  int rc = drain(c);
+  if (rc) return -rc;
+  return (int)c->inst[s].samples.size();
 }
-extern "C" int xpfc_inst_get_samples(const xpfc_ctx* c, int s, xpfc_sample* out, int max) {
-  if (!c || s < 0 || s >= c->K || (max > 0 && !out)) return fail(XPFC_ERR_ARG, "bad arguments");
+extern "C" int xpfc_inst_get_samples(xpfc_ctx* c, int s, xpfc_sample* out, int max) {
+  if (!c || s < 0 || s >= c->K || (max > 0 && !out)) return -fail(XPFC_ERR_ARG, "bad arguments");
+  int rc = drain(c);
+  if (rc) return -rc;
   const auto& v = c->inst[s].samples;
   const int n = std::min(max, (int)v.size());
   for (int i = 0; i < n; ++i) out[i] = v[i];
   return n;
 }
 extern "C" void xpfc_inst_clear_samples(xpfc_ctx* c, int s) {
-  if (c && s >= 0 && s < c->K) c->inst[s].samples.clear();
+  if (c && s >= 0 && s < c->K && drain(c) == XPFC_OK) c->inst[s].samples.clear();
 }
-extern "C" int xpfc_num_samples(const xpfc_ctx* c) { return xpfc_inst_num_samples(c, 0); }
-extern "C" int xpfc_get_samples(const xpfc_ctx* c, xpfc_sample* out, int max) {
+extern "C" int xpfc_num_samples(xpfc_ctx* c) { return xpfc_inst_num_samples(c, 0); }
+extern "C" int xpfc_get_samples(xpfc_ctx* c, xpfc_sample* out, int max) {
   return xpfc_inst_get_samples(c, 0, out, max);
 }
 extern "C" void xpfc_clear_samples(xpfc_ctx* c) { xpfc_inst_clear_samples(c, 0); }
@@ -1013,6 +1027,7 @@ extern "C" int xpfc_inst_sse_calc(xpfc_ctx* c, int s, double* rmse) {
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!rmse) return fail(XPFC_ERR_ARG, "null argument");
+  if ((rc = drain(c)) != XPFC_OK) return rc;
   if (!c->inst[s].have_rmse) return fail(XPFC_ERR_STATE, "no sampled step yet");
   *rmse = c->inst[s].last_rmse;
   return XPFC_OK;

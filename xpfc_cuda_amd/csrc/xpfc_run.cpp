@@ -63,8 +63,9 @@ int main(int argc, char** argv) {
     ms += t;
     done += nb;
     const int ns = xpfc_num_samples(ctx);
+    if (ns < 0) return die(-ns, "samples");
     buf.resize(ns > 0 ? ns : 1);
-    xpfc_get_samples(ctx, buf.data(), ns);
+    if ((rc = xpfc_get_samples(ctx, buf.data(), ns)) < 0) return die(-rc, "samples");
     for (; printed < ns; ++printed) {
       const xpfc_sample& s = buf[printed];
       if (s.has_fe) std::printf("iteration: %lld F = %0.10f", s.step, s.F_ref);
