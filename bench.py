@@ -7,7 +7,15 @@ A step is one full semi-implicit update of the field resident in HBM (the
 reference's loop body, main.cu:306-435); FE/RMSE are sampled every 100 steps.
 
 Prints one JSON line.  `value` = steps/s of the FP32-mixed path; the FP64
-plain path at the same size is timed beside it (`fp64_steps_per_s`).
+plain path at the same size is timed beside it (`fp64_steps_per_s`).  Two more
+workloads ride along in their own keys:
+  batched -- 64 independent 2-D square-lattice instances, sides drawn from
+             {128, 256, 512, 1024}, sigma ~ U[0.05, 0.2], n0 ~ U[0.02, 0.08],
+             noise amplitude 0.01 seeded by the instance id, FP32-mixed, 2000
+             steps each (FE/RMSE every 100) in 50-step graph blocks;
+  serving -- Poisson arrivals of 2-D (128^2-1024^2) and 3-D (64^3-256^3) jobs,
+             sides log-uniform, 500-5000 steps, admitted under a 200 GB budget;
+             p50/p99 time-to-completion and aggregate cell-updates/s.
 """
 from __future__ import annotations
 
@@ -85,6 +93,90 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
                       % (done, n, workers, platform.processor() or platform.machine())}
 
 
+def batched_run(steps=2000, ninst=64, seed=0):
+    """The 64-instance batch: one multi-instance context per grid side, all
+    classes queued at once on their own streams; wall time of the whole job."""
+    import numpy as np
+    import xpfc_cuda_amd as X
+    from xpfc_cuda_amd import serving as S
+    rng = np.random.default_rng(seed)
+    sides = rng.choice([128, 256, 512, 1024], size=ninst)
+    sig = rng.uniform(0.05, 0.2, ninst)
+    n0 = rng.uniform(0.02, 0.08, ninst)
+    groups = {}
+    for i in range(ninst):
+        groups.setdefault(int(sides[i]), []).append(i)
+    ctxs, cells = [], 0
+    for side, ids in sorted(groups.items()):
+        jobs = [S.Job(jid=i, dim=2, side=side, sigma=float(sig[i]), n0=float(n0[i]), steps=steps,
+                      seed=i) for i in ids]
+        ps = []
+        for j in jobs:
+            p = S.job_params(j)
+            p.t_sample = 100
+            ps.append(p)
+        bc = X.BatchContext(ps)
+        v = bc.autotune()
+        for s, j in enumerate(jobs):
+            bc.initialize_noise(s, j.amplitude, j.seed)
+        ctxs.append((side, ids, bc, v))
+        cells += side * side * len(ids)
+    for _, _, bc, _ in ctxs:
+        bc.sync()
+    t0 = time.perf_counter()
+    for _, _, bc, _ in ctxs:
+        bc.step(steps)          # queued: the classes overlap on their streams
+    for _, _, bc, _ in ctxs:
+        try:
+            bc.sync()
+        except X.XPFCError as e:
+            # strongly crystallising draws (small sigma, large n0) diverge under
+            # the explicit nonlinear term, as in the FP64 oracle; counted below
+            if e.status != X.api.XPFC_ERR_NONFINITE:
+                raise
+    wall = time.perf_counter() - t0
+    diverged = 0
+    fe = {}
+    for side, ids, bc, v in ctxs:
+        for s, i in enumerate(ids):
+            smp = bc.samples(s)
+            diverged += not all(math.isfinite(r.F_ref) for r in smp if r.has_fe)
+            if s == 0:
+                fe[str(side)] = [smp[-1].step, smp[-1].F_ref]
+    res = {
+        "instances": ninst, "steps": steps, "wall_s": round(wall, 4),
+        "instance_steps_per_s": round(ninst * steps / wall, 1),
+        "cell_updates_per_s": cells * steps / wall,
+        "classes": {str(side): {"instances": len(ids), "variants": list(v)}
+                    for side, ids, _, v in ctxs},
+        "last_fe_first_instance": fe, "nonfinite_instances": diverged,
+    }
+    for _, _, bc, _ in ctxs:
+        bc.close()
+    return res
+
+
+def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9):
+    """Poisson job stream through the continuous-batching service."""
+    from xpfc_cuda_amd import serving as S
+    jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d)
+    svc = S.Service(budget_bytes=budget)
+    try:
+        svc.warm([(2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128), (3, 256)])
+        rep = svc.run(jobs, realtime=True,
+                      progress=lambda m: print("serving " + m, file=sys.stderr, flush=True))
+        # offered load: cell-updates per second the arrivals ask for
+        offered = sum(j.cells * j.steps for j in jobs) / seconds
+        rep.update({"arrival_rate": rate, "window_s": seconds, "frac_3d": frac_3d,
+                    "offered_cell_updates_per_s": offered,
+                    "budget_gb": budget / 1e9,
+                    "rounding": "side log-uniform, rounded to the nearest power of two in log2",
+                    "steps": "U{500..5000}", "sigma": "U[0.05,0.2]", "n0": "U[0.02,0.08]"})
+    finally:
+        svc.close()
+    return rep
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -93,6 +185,13 @@ def main():
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve", type=float, default=30.0,
+                    help="seconds of Poisson arrivals for the serving leg (0 = skip)")
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-3d", type=float, default=0.5, help="fraction of 3-D jobs")
+    ap.add_argument("--only", choices=["batch", "serve"],
+                    help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run --steps unsampled steps and one field read-back (its D2D copy of "
                          "the packed spectrum is the known-size copy for counter calibration)")
@@ -113,6 +212,13 @@ def main():
               "(the workload would shard as x-slabs with two all-to-all transposes per step)",
               file=sys.stderr)
         sys.exit(2)
+    if args.only == "batch":
+        print(json.dumps({"batched": batched_run()}), flush=True)
+        return
+    if args.only == "serve":
+        print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d)}),
+              flush=True)
+        return
     n = args.n
     ms, wall, passes, samples = gpu_run(n, 1, args.steps, args.warmup)
     ms_per_step = ms / args.steps
@@ -156,6 +262,15 @@ def main():
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
         out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
+    tp = os.path.join(ROOT, "profiles", "traffic_512cube_fp32.json")
+    if n == 512 and os.path.exists(tp):
+        # HBM bytes per k_xstep launch from the committed FETCH_SIZE/WRITE_SIZE runs
+        with open(tp) as f:
+            out["roofline"]["traffic"] = json.load(f)["k_xstep"]["traffic_bytes"]
+    if not args.no_batch:
+        out["batched"] = batched_run()
+    if args.serve > 0:
+        out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

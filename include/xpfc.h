@@ -158,6 +158,9 @@ int xpfc_get_variants(const xpfc_ctx* ctx, int* v);
  * 50 steps and replayed (on by default; XPFC_GRAPHS=0 or on = 0 launches every
  * kernel eagerly).  Both give identical fields. */
 int xpfc_set_graphs(xpfc_ctx* ctx, int on);
+/* 1 when all work queued on the context has finished, 0 while it runs
+ * (never blocks), minus a status code on failure. */
+int xpfc_query(xpfc_ctx* ctx);
 /* Block until all work of the context has finished; read back pending samples. */
 int xpfc_sync(xpfc_ctx* ctx);
 
@@ -212,6 +215,9 @@ int xpfc_read_vtk(const char* path, int* nx, int* ny, double* spacing, double* h
  * a one-instance context given the same parameters and field.  The functions
  * above act on instance 0. */
 int xpfc_create_batch(const xpfc_params* p, int ninst, xpfc_ctx** out);  /* p[ninst] */
+/* Device bytes xpfc_create_batch allocates per instance of this grid and
+ * precision (0 for an unsupported grid): the unit of HBM admission. */
+size_t xpfc_instance_bytes(const xpfc_params* p);
 int xpfc_num_instances(const xpfc_ctx* ctx);
 /* Active list (NULL = all instances, in order); stepping skips the others. */
 int xpfc_set_active(xpfc_ctx* ctx, const int* inst, int n);

@@ -14,6 +14,15 @@ XPFC_FP32_MIXED = 1
 XPFC_DCF_SUM = 0
 XPFC_DCF_MAX = 1
 MAX_PEAKS = 4
+# status codes (include/xpfc.h)
+XPFC_OK = 0
+XPFC_ERR_ARG = 1
+XPFC_ERR_IO = 2
+XPFC_ERR_PARSE = 3
+XPFC_ERR_DEVICE = 4
+XPFC_ERR_NOMEM = 5
+XPFC_ERR_STATE = 6
+XPFC_ERR_NONFINITE = 7
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 
@@ -113,6 +122,8 @@ def load_library(path: Optional[str] = None):
         "xpfc_read_vtk": (i, [ct.c_char_p, P(i), P(i), P(d), P(d), ll]),
         "xpfc_create_batch": (i, [P(_Params), i, P(vp)]),
         "xpfc_num_instances": (i, [vp]),
+        "xpfc_instance_bytes": (sz, [P(_Params)]),
+        "xpfc_query": (i, [vp]),
         "xpfc_set_active": (i, [vp, P(i), i]),
         "xpfc_inst_set_params": (i, [vp, i, P(_Params)]),
         "xpfc_inst_reset": (i, [vp, i]),
@@ -252,6 +263,12 @@ def read_input(path: str) -> Params:
 def reference_steps(p: Params) -> int:
     c = p.to_c()
     return int(lib().xpfc_reference_steps(ct.byref(c)))
+
+
+def instance_bytes(p: Params) -> int:
+    """Device bytes one instance of this grid and precision occupies."""
+    c = p.to_c()
+    return int(lib().xpfc_instance_bytes(ct.byref(c)))
 
 
 def randn(seed: int, count: int, first: int = 0) -> np.ndarray:
@@ -528,6 +545,13 @@ class BatchContext:
         h = ct.c_void_p()
         _check(lib().xpfc_create_batch(arr, len(self.params), ct.byref(h)), "xpfc_create_batch")
         self.handle = h
+
+    def query(self) -> bool:
+        """True when every queued step has finished (never blocks)."""
+        r = lib().xpfc_query(self.handle)
+        if r < 0:
+            _check(-r, "xpfc_query")
+        return r == 1
 
     close = Context.close
     __del__ = Context.__del__

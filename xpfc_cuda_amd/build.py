@@ -60,7 +60,7 @@ def build(verbose: bool = False) -> str:
                 print(out)
     lib = os.path.join(LIBDIR, "libxpfc.so")
     tmp = lib + ".tmp"
-    _run([hipcc, *HIPFLAGS, "-shared", "-o", tmp, *objs])
+    _run([hipcc, *HIPFLAGS, "-shared", "-pthread", "-o", tmp, *objs])
     os.replace(tmp, lib)
     cli_src = os.path.join(CSRC, "xpfc_run.cpp")
     if os.path.exists(cli_src):

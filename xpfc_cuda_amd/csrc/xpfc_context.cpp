@@ -64,8 +64,8 @@ int default_tc(int prec, int L, const Segs& s) {
   return -1;
 }
 
-// Captured step blocks bake in buffer pointers, pass variants and the active
-// instance count: drop them whenever one of those changes.
+// Captured step blocks bake in buffer pointers and pass variants (the active
+// count is part of the cache key): drop them whenever one of those changes.
 void drop_graphs(xpfc_ctx* c) {
   for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.second);
   c->graphs.clear();
@@ -273,6 +273,29 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
 
 extern "C" int xpfc_create(const xpfc_params* p, xpfc_ctx** out) { return xpfc_create_batch(p, 1, out); }
 
+extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
+  // what xpfc_create_batch allocates per instance: three spectra, the partial
+  // sums of the largest pass variants, the sampled-sum rows
+  if (!p || (p->dim != 2 && p->dim != 3) || (p->precision != XPFC_FP64 && p->precision != XPFC_FP32_MIXED))
+    return 0;
+  const int prec = p->precision == XPFC_FP32_MIXED ? 1 : 0;
+  const int n0 = p->n[0], n1 = p->dim == 3 ? p->n[1] : 1, n2 = p->dim == 3 ? p->n[2] : p->n[1];
+  for (int v : {n0, n1, n2})
+    if (v < 1 || !is_pow2(v)) return 0;
+  const int H = n2 / 2;
+  if (H < 1) return 0;
+  const long long nmain = (long long)n0 * n1 * H, nny = (long long)n0 * n1;
+  const size_t csize = prec == 1 ? 8 : 16;
+  Segs sx{};
+  sx.s[0] = Seg{0, 0, n1 * H, 1, n1 * H, 1, 0, 0};
+  sx.s[1] = Seg{nmain, 0, n1, 1, n1, 1, 0, 1};
+  int npx = 4096, npz = 1;
+  for (int tc : {4, 8, 16}) npx = std::max(npx, strided_blocks(prec, n0, tc, sx));
+  for (int nz : {64, 128, 256, 512}) npz = std::max(npz, zpass_blocks(prec, H, nz, (long long)n0 * n1));
+  return 3 * (size_t)(nmain + nny) * csize + ((size_t)npx * XPART + (size_t)npz * ZPART) * sizeof(double) +
+         (size_t)(SUM_ROWS + 1) * NSUMS * sizeof(double) + 2 * sizeof(InstArgs) + 3 * sizeof(int);
+}
+
 extern "C" void xpfc_destroy(xpfc_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
@@ -340,7 +363,6 @@ extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
     HIPCHK(hipStreamSynchronize(c->stream));
   }
   c->active = a;
-  drop_graphs(c);
   return XPFC_OK;
 }
 
@@ -559,16 +581,19 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
   return XPFC_OK;
 }
 
-// Unsampled steps run as replays of captured hipGraphs of up to GRAPH_BLOCK
-// steps (one graph per run length, cached per context).  The captured work is
-// the same launch sequence as an eager step, so the fields are identical.
+// Unsampled steps run as replays of captured hipGraphs of GRAPH_BLOCK steps
+// or a power of two below it, cached per context by (length, number of active
+// instances): the kernels read the active list from device memory, so a new
+// list of the same length reuses the graphs.  The captured work is the same
+// launch sequence as an eager step, so the fields are identical.
 int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
+  const int ny = (int)c->active.size();
   for (auto& g : c->graphs)
-    if (g.first == len) {
+    if (g.first.first == len && g.first.second == ny) {
       *out = g.second;
       return XPFC_OK;
     }
-  if (c->graphs.size() >= 8) drop_graphs(c);
+  if (c->graphs.size() >= 32) drop_graphs(c);
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = XPFC_OK;
@@ -582,7 +607,7 @@ int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
   const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
-  c->graphs.emplace_back(len, ex);
+  c->graphs.push_back({{len, ny}, ex});
   *out = ex;
   return XPFC_OK;
 }
@@ -600,7 +625,11 @@ int run_plain_steps(xpfc_ctx* c, long long run) {
       --run;
       continue;
     }
-    const int len = (int)std::min<long long>(run, GRAPH_BLOCK);
+    int len = GRAPH_BLOCK;
+    if (run < GRAPH_BLOCK) {
+      len = 1;
+      while (2 * len <= run) len *= 2;
+    }
     hipGraphExec_t ex = nullptr;
     int rc = plain_graph(c, len, &ex);
     if (rc) return rc;
@@ -969,6 +998,14 @@ extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
   ms[4] = (double)tot / nsteps;
   for (auto& e : ev) (void)hipEventDestroy(e);
   return XPFC_OK;
+}
+
+extern "C" int xpfc_query(xpfc_ctx* c) {
+  if (!c) return -fail(XPFC_ERR_ARG, "null context");
+  const hipError_t e = hipStreamQuery(c->stream);
+  if (e == hipSuccess) return 1;
+  if (e == hipErrorNotReady) return 0;
+  return -fail(XPFC_ERR_DEVICE, "hipStreamQuery failed: %s", hipGetErrorString(e));
 }
 
 extern "C" int xpfc_sync(xpfc_ctx* c) {

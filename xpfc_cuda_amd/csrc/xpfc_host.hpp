@@ -87,7 +87,7 @@ struct xpfc_ctx {
   int tc_x = 0, tc_y = 0, ntz = 256;  // pass variants
   bool tuned = false;
   bool use_graphs = true;
-  std::vector<std::pair<int, hipGraphExec_t>> graphs;
+  std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;  // (steps, active) -> graph
   std::vector<XInst> inst;
   std::vector<int> active;
   std::vector<XPending> pending;

@@ -11,6 +11,7 @@
 #include <cstring>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 namespace xpfc {
@@ -271,13 +272,12 @@ extern "C" void xpfc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2],
   out[3] = c3;
 }
 
-extern "C" int xpfc_randn(unsigned long long seed, unsigned long long first, long long count,
-                          double* out) {
-  if (count < 0 || (count > 0 && !out)) return fail(XPFC_ERR_ARG, "bad randn arguments");
-  const uint32_t key[2] = {(uint32_t)(seed & 0xFFFFFFFFu), (uint32_t)(seed >> 32)};
+namespace {
+void randn_range(const uint32_t key[2], unsigned long long first, long long i0, long long i1,
+                 double* out) {
   const double two_pi = 2.0 * M_PI;
   const double s32 = 1.0 / 4294967296.0;
-  for (long long i = 0; i < count; ++i) {
+  for (long long i = i0; i < i1; ++i) {
     const unsigned long long c = first + (unsigned long long)i;
     const uint32_t ctr[4] = {(uint32_t)(c & 0xFFFFFFFFu), (uint32_t)(c >> 32), 0u, 0u};
     uint32_t r[4];
@@ -286,6 +286,29 @@ extern "C" int xpfc_randn(unsigned long long seed, unsigned long long first, lon
     const double u1 = ((double)r[1] + 0.5) * s32;
     out[i] = std::sqrt(-2.0 * std::log(u0)) * std::cos(two_pi * u1);
   }
+}
+}  // namespace
+
+// Each draw depends only on (seed, cell), so large requests are split over
+// host threads with identical results.
+extern "C" int xpfc_randn(unsigned long long seed, unsigned long long first, long long count,
+                          double* out) {
+  if (count < 0 || (count > 0 && !out)) return fail(XPFC_ERR_ARG, "bad randn arguments");
+  const uint32_t key[2] = {(uint32_t)(seed & 0xFFFFFFFFu), (uint32_t)(seed >> 32)};
+  const long long chunk = 1LL << 19;
+  const int nt = (int)std::min<long long>({(count + chunk - 1) / chunk, 16LL,
+                                           (long long)std::max(1u, std::thread::hardware_concurrency())});
+  if (nt <= 1) {
+    randn_range(key, first, 0, count, out);
+    return XPFC_OK;
+  }
+  std::vector<std::thread> th;
+  const long long per = (count + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const long long i0 = t * per, i1 = std::min(count, i0 + per);
+    if (i0 < i1) th.emplace_back(randn_range, key, first, i0, i1, out);
+  }
+  for (auto& x : th) x.join();
   return XPFC_OK;
 }
 
