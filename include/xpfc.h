@@ -73,6 +73,7 @@ typedef struct xpfc_params {
   int iseed, ishape;
   double h, k, roa;
   int device;                /* HIP device ordinal                              */
+  int restart_order;         /* VTK restart mapping, see xpfc_inst_initialize_vtk */
 } xpfc_params;
 
 /* One sampled diagnostic record (what sum_reduction.cu:64,77 prints). */
@@ -161,6 +162,11 @@ int xpfc_set_graphs(xpfc_ctx* ctx, int on);
 /* 1 when all work queued on the context has finished, 0 while it runs
  * (never blocks), minus a status code on failure. */
 int xpfc_query(xpfc_ctx* ctx);
+/* Divergence probe after the queued steps (blocks until they finish):
+ * finite[s] = 1 while instance s's field is finite, 0 once it has diverged
+ * (two spectral modes are tested; a non-finite value anywhere in the field
+ * reaches every mode within one step).  finite[ninst]. */
+int xpfc_probe_finite(xpfc_ctx* ctx, int* finite);
 /* Block until all work of the context has finished; read back pending samples. */
 int xpfc_sync(xpfc_ctx* ctx);
 
@@ -201,10 +207,27 @@ int xpfc_fft_c2r(xpfc_ctx* ctx, const void* spec_in, void* real_out);
 
 /* ---- output (write_vtk.cu:4-35) ------------------------------------------- */
 int xpfc_write_vtk(xpfc_ctx* ctx, const char* path);
-/* Restart input (read_input.cu:161-241): read a 2-D legacy VTK field into
- * host memory (values in file order), dims returned. */
-int xpfc_read_vtk(const char* path, int* nx, int* ny, double* spacing, double* host_real,
+/* Restart input (read_input.cu:124-241): read a legacy VTK structured-points
+ * field (2-D or 3-D) into host memory, values in file order (x fastest);
+ * dims[3] and spacing[3] returned; host_real may be NULL to query dims. */
+int xpfc_read_vtk(const char* path, int* dims, double* spacing, double* host_real,
                   long long capacity);
+/* Initial condition from a VTK file written by xpfc_write_vtk (or the
+ * reference's writeVTKFile), with the average-density fix.  p->restart_order
+ * selects how file values map to cells: 0 = x fastest (the file's own order;
+ * write -> read is the identity), 1 = the reference's sequential fill
+ * psi[i*My + j] (read_input.cu:208-218; transposes x and y, 2-D only).
+ * xpfc_initialize with iconfig = 10 reads "initial_density_field.vtk" from the
+ * working directory like the reference. */
+int xpfc_inst_initialize_vtk(xpfc_ctx* ctx, int s, const char* path);
+/* Binary checkpoint of one instance: parameters, step, FP64 mean and the
+ * device spectra (the state and the nonlinear-term work field), so a run
+ * resumed from it continues bit for bit.  Loading requires the same grid and
+ * precision; the sample log is cleared. */
+int xpfc_inst_save_checkpoint(xpfc_ctx* ctx, int s, const char* path);
+int xpfc_inst_load_checkpoint(xpfc_ctx* ctx, int s, const char* path);
+int xpfc_save_checkpoint(xpfc_ctx* ctx, const char* path);
+int xpfc_load_checkpoint(xpfc_ctx* ctx, const char* path);
 
 /* ---- several simulations of one grid in one context ------------------------ */
 /* No counterpart in the reference, which runs one simulation per process

@@ -140,3 +140,48 @@ def test_batch_rejects_mismatched_grid():
         X.BatchContext([p, r])
     with pytest.raises(ValueError):
         X.BatchContext([])
+
+
+def _vtk3_text(psi, L):
+    """A 3-D legacy VTK file laid out like the 2-D writer (x fastest)."""
+    nx, ny, nz = psi.shape
+    lines = ["# vtk DataFile Version 3.0", "Density Data", "ASCII", "DATASET STRUCTURED_POINTS",
+             "DIMENSIONS %d %d %d" % (nx, ny, nz), "ORIGIN 0 0 0",
+             "SPACING %g %g %g" % (L[0] / nx, L[1] / ny, L[2] / nz),
+             "POINT_DATA %d" % psi.size, "SCALARS DensityData float", "LOOKUP_TABLE default"]
+    lines += ["%g" % v for v in psi.transpose(2, 1, 0).reshape(-1)]
+    return "\n".join(lines) + "\n"
+
+
+def test_read_vtk_2d_and_3d(tmp_path):
+    rng = np.random.default_rng(0)
+    psi2 = 0.05 + 0.01 * rng.standard_normal((6, 4))
+    f2 = tmp_path / "a.vtk"
+    f2.write_text(O.vtk_text(psi2, 12, 8))
+    dims, sp, v = X.api.read_vtk(str(f2))
+    assert dims == (6, 4, 1) and sp == (2.0, 2.0, 1.0)
+    # file order is x fastest
+    expect = np.array([float("%g" % x) for x in psi2.T.reshape(-1)])
+    assert np.array_equal(v, expect)
+    psi3 = rng.standard_normal((4, 3, 2))
+    f3 = tmp_path / "b.vtk"
+    f3.write_text(_vtk3_text(psi3, (4.0, 3.0, 2.0)))
+    dims, sp, v = X.api.read_vtk(str(f3))
+    assert dims == (4, 3, 2) and sp == (1.0, 1.0, 1.0)
+    assert np.array_equal(v.reshape(2, 3, 4).transpose(2, 1, 0),
+                          np.vectorize(lambda x: float("%g" % x))(psi3))
+    (tmp_path / "c.vtk").write_text("# vtk\nx\n")
+    with pytest.raises(X.XPFCError):
+        X.api.read_vtk(str(tmp_path / "c.vtk"))
+
+
+def test_read_input_restart_box_from_vtk(tmp_path, monkeypatch):
+    """iconfig 10 takes the box from the restart file (read_input.cu:124-157)."""
+    text = PFC_INPUT.replace("1          iconfig", "10         iconfig")
+    assert text != PFC_INPUT
+    (tmp_path / "pfc.input").write_text(text)
+    psi = np.zeros((64, 32))
+    (tmp_path / "initial_density_field.vtk").write_text(O.vtk_text(psi, 40, 24))
+    monkeypatch.chdir(tmp_path)
+    p = X.read_input(str(tmp_path / "pfc.input"))
+    assert p.iconfig == 10 and p.L[:2] == (40.0, 24.0)

@@ -49,8 +49,9 @@ def test_alone_equals_mix():
             c.step(job.steps)
             alone = c.get_field()
             if job.jid == 4:
+                # stopped at the end of the block it diverged in
                 assert job.status == "nonfinite" and not np.all(np.isfinite(alone))
-                assert np.array_equal(job.field, alone, equal_nan=True)
+                assert not np.all(np.isfinite(job.field))
                 continue
             F, fm = c.fe_calc()
             m = c.avg_calc()

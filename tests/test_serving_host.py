@@ -47,6 +47,9 @@ class FakeCtx:
             self.busy_polls -= 1
         return self.busy_polls == 0
 
+    def probe_finite(self):
+        return [True] * self.n
+
     def fe_calc(self, s):
         return float(self.steps[s]), 0.0
 
@@ -130,3 +133,27 @@ def test_poisson_stream_statistics():
     assert f == pytest.approx(0.5, abs=0.04)
     assert all(500 <= j.steps <= 5000 for j in jobs)
     assert all(0.05 <= j.sigma <= 0.2 and 0.02 <= j.n0 <= 0.08 for j in jobs)
+
+
+def test_diverged_job_stops_early_and_frees_its_slot(fake, monkeypatch):
+    seeds = {}
+
+    def init(self, s, amp, seed):
+        seeds[s] = seed
+        FakeCtx.log.append(("init", s, seed))
+
+    # the job with seed 1 'diverges' once it has taken 60 steps
+    monkeypatch.setattr(FakeCtx, "initialize_noise", init)
+    monkeypatch.setattr(FakeCtx, "probe_finite",
+                        lambda self: [not (seeds.get(s) == 1 and self.steps[s] >= 60)
+                                      for s in range(self.n)])
+    jobs = [_job(0, 64, 300), _job(1, 64, 1000), _job(2, 64, 80)]
+    svc = S.Service(budget_bytes=1e12, slots=lambda d, s: 2)
+    rep = svc.run(jobs, realtime=False, idle_sleep=0)
+    by = {j.jid: j for j in svc.done}
+    assert rep["nonfinite_jobs"] == 1 and by[1].status == "nonfinite"
+    assert by[0].status == by[2].status == "done"
+    assert by[0].F_ref == 300 and by[2].F_ref == 80
+    # job 2 took job 1's slot long before job 1's 1000 steps would have ended
+    assert by[2].admitted >= by[1].finished
+    assert by[2].finished < by[0].finished

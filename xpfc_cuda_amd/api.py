@@ -53,7 +53,7 @@ class _Params(ct.Structure):
                 ("A_10", ct.c_double), ("A_11", ct.c_double), ("A_tri", ct.c_double),
                 ("iseed", ct.c_int), ("ishape", ct.c_int),
                 ("h", ct.c_double), ("k", ct.c_double), ("roa", ct.c_double),
-                ("device", ct.c_int)]
+                ("device", ct.c_int), ("restart_order", ct.c_int)]
 
 
 class _Sample(ct.Structure):
@@ -119,11 +119,17 @@ def load_library(path: Optional[str] = None):
         "xpfc_fft_r2c": (i, [vp, vp, vp]),
         "xpfc_fft_c2r": (i, [vp, vp, vp]),
         "xpfc_write_vtk": (i, [vp, ct.c_char_p]),
-        "xpfc_read_vtk": (i, [ct.c_char_p, P(i), P(i), P(d), P(d), ll]),
+        "xpfc_read_vtk": (i, [ct.c_char_p, P(i), P(d), P(d), ll]),
+        "xpfc_inst_initialize_vtk": (i, [vp, i, ct.c_char_p]),
+        "xpfc_inst_save_checkpoint": (i, [vp, i, ct.c_char_p]),
+        "xpfc_inst_load_checkpoint": (i, [vp, i, ct.c_char_p]),
+        "xpfc_save_checkpoint": (i, [vp, ct.c_char_p]),
+        "xpfc_load_checkpoint": (i, [vp, ct.c_char_p]),
         "xpfc_create_batch": (i, [P(_Params), i, P(vp)]),
         "xpfc_num_instances": (i, [vp]),
         "xpfc_instance_bytes": (sz, [P(_Params)]),
         "xpfc_query": (i, [vp]),
+        "xpfc_probe_finite": (i, [vp, P(i)]),
         "xpfc_set_active": (i, [vp, P(i), i]),
         "xpfc_inst_set_params": (i, [vp, i, P(_Params)]),
         "xpfc_inst_reset": (i, [vp, i]),
@@ -195,6 +201,7 @@ class Params:
     k: float = 0.0
     roa: float = 0.0
     device: int = 0
+    restart_order: int = 0
 
     @property
     def dim(self) -> int:
@@ -211,7 +218,7 @@ class Params:
                   "h", "k", "roa"):
             setattr(c, f, float(getattr(self, f)))
         for f in ("dcf_mode", "precision", "t_sample", "int_Mo_residual", "no_val", "t_final",
-                  "iconfig", "iseed", "ishape", "device"):
+                  "iconfig", "iseed", "ishape", "device", "restart_order"):
             setattr(c, f, int(getattr(self, f)))
         if not 1 <= len(self.peaks) <= MAX_PEAKS:
             raise ValueError("1..4 DCF peaks")
@@ -230,7 +237,8 @@ class Params:
                    dcf_mode=c.dcf_mode, precision=c.precision, t_sample=c.t_sample,
                    int_Mo_residual=c.int_Mo_residual, no_val=c.no_val, t_final=c.t_final,
                    iconfig=c.iconfig, A_10=c.A_10, A_11=c.A_11, A_tri=c.A_tri, iseed=c.iseed,
-                   ishape=c.ishape, h=c.h, k=c.k, roa=c.roa, device=c.device)
+                   ishape=c.ishape, h=c.h, k=c.k, roa=c.roa, device=c.device,
+                   restart_order=c.restart_order)
 
 
 @dataclass
@@ -263,6 +271,17 @@ def read_input(path: str) -> Params:
 def reference_steps(p: Params) -> int:
     c = p.to_c()
     return int(lib().xpfc_reference_steps(ct.byref(c)))
+
+
+def read_vtk(path: str):
+    """Legacy VTK structured points -> (dims, spacing, values in file order)."""
+    dims = (ct.c_int * 3)()
+    sp = (ct.c_double * 3)()
+    _check(lib().xpfc_read_vtk(path.encode(), dims, sp, None, 0), "xpfc_read_vtk")
+    n = dims[0] * dims[1] * dims[2]
+    out = np.empty(n, dtype=np.float64)
+    _check(lib().xpfc_read_vtk(path.encode(), dims, sp, _dptr(out), n), "xpfc_read_vtk")
+    return tuple(dims), tuple(sp), out
 
 
 def instance_bytes(p: Params) -> int:
@@ -462,6 +481,17 @@ class Context:
         """writeVTKFile (write_vtk.cu:4-35)."""
         _check(lib().xpfc_write_vtk(self.handle, path.encode()), "xpfc_write_vtk")
 
+    def initialize_vtk(self, path: str):
+        """read_density_field (read_input.cu:161-241) + the average-density fix."""
+        _check(lib().xpfc_inst_initialize_vtk(self.handle, 0, path.encode()),
+               "xpfc_inst_initialize_vtk")
+
+    def save_checkpoint(self, path: str):
+        _check(lib().xpfc_save_checkpoint(self.handle, path.encode()), "xpfc_save_checkpoint")
+
+    def load_checkpoint(self, path: str):
+        _check(lib().xpfc_load_checkpoint(self.handle, path.encode()), "xpfc_load_checkpoint")
+
     # --- operator level (host arrays in, host arrays out; device buffers inside)
     def _buf(self, a: np.ndarray) -> DeviceBuffer:
         b = DeviceBuffer(self, a.nbytes)
@@ -553,6 +583,12 @@ class BatchContext:
             _check(-r, "xpfc_query")
         return r == 1
 
+    def probe_finite(self) -> List[bool]:
+        """Per instance: False once its field has diverged (waits for queued steps)."""
+        v = (ct.c_int * self.ninst)()
+        _check(lib().xpfc_probe_finite(self.handle, v), "xpfc_probe_finite")
+        return [bool(x) for x in v]
+
     close = Context.close
     __del__ = Context.__del__
     __enter__ = Context.__enter__
@@ -641,3 +677,15 @@ class BatchContext:
 
     def write_vtk(self, s: int, path: str):
         _check(lib().xpfc_inst_write_vtk(self.handle, s, path.encode()), "xpfc_inst_write_vtk")
+
+    def initialize_vtk(self, s: int, path: str):
+        _check(lib().xpfc_inst_initialize_vtk(self.handle, s, path.encode()),
+               "xpfc_inst_initialize_vtk")
+
+    def save_checkpoint(self, s: int, path: str):
+        _check(lib().xpfc_inst_save_checkpoint(self.handle, s, path.encode()),
+               "xpfc_inst_save_checkpoint")
+
+    def load_checkpoint(self, s: int, path: str):
+        _check(lib().xpfc_inst_load_checkpoint(self.handle, s, path.encode()),
+               "xpfc_inst_load_checkpoint")

@@ -74,7 +74,7 @@ void drop_graphs(xpfc_ctx* c) {
 void free_all(xpfc_ctx* c) {
   drop_graphs(c);
   void* ptrs[] = {c->psi, c->work, c->aux, c->tw0, c->tw1, c->twz, c->twp, c->px,
-                  c->pz, c->sums, c->d_ia, c->d_ids, c->d_active};
+                  c->pz, c->sums, c->d_ia, c->d_ids, c->d_active, c->d_flags};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -240,7 +240,8 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
       hipMalloc((void**)&c->sums, (size_t)(SUM_ROWS + 1) * K * NSUMS * sizeof(double)) != hipSuccess ||
       hipMalloc((void**)&c->d_ia, (size_t)(K + 1) * sizeof(InstArgs)) != hipSuccess ||
       hipMalloc((void**)&c->d_ids, (size_t)(K + 1) * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&c->d_active, (size_t)K * sizeof(int)) != hipSuccess)
+      hipMalloc((void**)&c->d_active, (size_t)K * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_flags, (size_t)K * sizeof(int)) != hipSuccess)
     return bail(fail(XPFC_ERR_NOMEM, "table allocation failed"));
 
   c->g.n0 = c->n0;
@@ -485,12 +486,7 @@ extern "C" int xpfc_inst_initialize(xpfc_ctx* c, int s) {
   const xpfc_params& p = c->inst[s].p;
   std::vector<double> psi((size_t)c->N, p.psi0);
   if (p.iconfig == 10) {
-    int nx = 0, ny = 0;
-    double sp[3];
-    std::vector<double> v;
-    if ((rc = read_vtk_file("initial_density_field.vtk", &nx, &ny, sp, &v))) return rc;
-    if ((long long)v.size() != c->N) return fail(XPFC_ERR_ARG, "VTK field size does not match the grid");
-    psi = v;  // read_input.cu:208-218 fills psi[i*My + j] in file order
+    return xpfc_inst_initialize_vtk(c, s, "initial_density_field.vtk");
   } else if (p.dim == 2) {
     const int Mx = p.n[0], My = p.n[1];
     const auto x = linspace_ref(0.0, p.L[0], Mx);
@@ -529,6 +525,35 @@ extern "C" int xpfc_inst_initialize(xpfc_ctx* c, int s) {
     return fail(XPFC_ERR_ARG, "3-D lattice initial conditions are not defined; use liquid (2) or noise");
   }
   fix_mean(psi, p.psi0);
+  return xpfc_inst_set_field(c, s, psi.data());
+}
+
+extern "C" int xpfc_inst_initialize_vtk(xpfc_ctx* c, int s, const char* path) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!path) return fail(XPFC_ERR_ARG, "null path");
+  const xpfc_params& p = c->inst[s].p;
+  int dims[3];
+  double sp[3];
+  std::vector<double> v;
+  if ((rc = read_vtk_file(path, dims, sp, &v))) return rc;
+  const int nz = p.dim == 3 ? p.n[2] : 1;
+  if (dims[0] != p.n[0] || dims[1] != p.n[1] || dims[2] != nz)
+    return fail(XPFC_ERR_ARG, "%s: dimensions %d %d %d do not match the grid %d %d %d", path, dims[0],
+                dims[1], dims[2], p.n[0], p.n[1], nz);
+  std::vector<double> psi((size_t)c->N);
+  if (p.restart_order == 1) {
+    if (p.dim != 2) return fail(XPFC_ERR_ARG, "restart_order 1 (reference fill) is 2-D only");
+    psi = v;  // read_input.cu:208-218: psi[i*My + j] filled in file order
+  } else {
+    // file order is x fastest (write_vtk.cu:25-27); cells are [i][j][l], l fastest
+    const int nx = dims[0], ny = dims[1];
+    for (int l = 0; l < nz; ++l)
+      for (int j = 0; j < ny; ++j)
+        for (int i = 0; i < nx; ++i)
+          psi[((size_t)i * ny + j) * nz + l] = v[((size_t)l * ny + j) * nx + i];
+  }
+  fix_mean(psi, p.psi0);  // initialize.cu:200-202
   return xpfc_inst_set_field(c, s, psi.data());
 }
 
@@ -1008,6 +1033,15 @@ extern "C" int xpfc_query(xpfc_ctx* c) {
   return -fail(XPFC_ERR_DEVICE, "hipStreamQuery failed: %s", hipGetErrorString(e));
 }
 
+extern "C" int xpfc_probe_finite(xpfc_ctx* c, int* finite) {
+  if (!c || !finite) return fail(XPFC_ERR_ARG, "null argument");
+  LCHK(launch_probe(c->prec, c->psi, c->stride, c->nmain, c->K, c->d_flags, c->stream));
+  HIPCHK(hipMemcpyAsync(finite, c->d_flags, (size_t)c->K * sizeof(int), hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return XPFC_OK;
+}
+
 extern "C" int xpfc_sync(xpfc_ctx* c) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
@@ -1224,3 +1258,99 @@ extern "C" int xpfc_inst_write_vtk(xpfc_ctx* c, int s, const char* path) {
 }
 
 extern "C" int xpfc_write_vtk(xpfc_ctx* c, const char* path) { return xpfc_inst_write_vtk(c, 0, path); }
+
+// ---------------------------------------------------------------------------
+// binary checkpoints: header, then the state spectrum and the work spectrum of
+// the instance exactly as they sit in device memory (packed half spectrum)
+// ---------------------------------------------------------------------------
+namespace {
+struct CkptHeader {
+  char magic[8];          // "XPFCCKP1"
+  int32_t version;        // 1
+  int32_t precision;
+  int32_t n0, n1, n2;     // packed-layout extents
+  int64_t step;
+  double mean;
+  int64_t stride;         // complex elements per spectrum
+  int64_t csize;          // bytes per complex element
+  xpfc_params params;
+};
+const char kMagic[8] = {'X', 'P', 'F', 'C', 'C', 'K', 'P', '1'};
+}  // namespace
+
+extern "C" int xpfc_inst_save_checkpoint(xpfc_ctx* c, int s, const char* path) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!path) return fail(XPFC_ERR_ARG, "null path");
+  if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
+  const XInst& in = c->inst[s];
+  CkptHeader h{};
+  std::memcpy(h.magic, kMagic, 8);
+  h.version = 1;
+  h.precision = c->prec;
+  h.n0 = c->n0;
+  h.n1 = c->n1;
+  h.n2 = c->n2;
+  h.step = in.step;
+  h.mean = in.mean;
+  h.stride = c->stride;
+  h.csize = (int64_t)c->csize;
+  h.params = in.p;
+  const size_t bytes = (size_t)c->stride * c->csize;
+  std::vector<char> buf(2 * bytes);
+  HIPCHK(hipMemcpyAsync(buf.data(), inst_ptr(c, c->psi, s), bytes, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(buf.data() + bytes, inst_ptr(c, c->work, s), bytes, hipMemcpyDeviceToHost,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  FILE* f = std::fopen(path, "wb");
+  if (!f) return fail(XPFC_ERR_IO, "cannot open %s for writing", path);
+  const bool ok = std::fwrite(&h, sizeof(h), 1, f) == 1 && std::fwrite(buf.data(), 1, buf.size(), f) == buf.size();
+  if (std::fclose(f) != 0 || !ok) return fail(XPFC_ERR_IO, "write to %s failed", path);
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!path) return fail(XPFC_ERR_ARG, "null path");
+  FILE* f = std::fopen(path, "rb");
+  if (!f) return fail(XPFC_ERR_IO, "cannot open %s", path);
+  CkptHeader h{};
+  const size_t bytes = (size_t)c->stride * c->csize;
+  std::vector<char> buf(2 * bytes);
+  bool ok = std::fread(&h, sizeof(h), 1, f) == 1;
+  ok = ok && std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 1;
+  if (!ok) {
+    std::fclose(f);
+    return fail(XPFC_ERR_PARSE, "%s: not an XPFC checkpoint", path);
+  }
+  if (h.precision != c->prec || h.n0 != c->n0 || h.n1 != c->n1 || h.n2 != c->n2 ||
+      h.stride != c->stride || h.csize != (int64_t)c->csize) {
+    std::fclose(f);
+    return fail(XPFC_ERR_ARG, "%s: grid or precision differ from the context's", path);
+  }
+  ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
+  std::fclose(f);
+  if (!ok) return fail(XPFC_ERR_PARSE, "%s: truncated", path);
+  if ((rc = drain(c)) != XPFC_OK) return rc;
+  XInst& in = c->inst[s];
+  in.mean = h.mean;
+  if ((rc = build_inst(c, h.params, &in)) != XPFC_OK) return rc;
+  if ((rc = upload_inst(c, s)) != XPFC_OK) return rc;
+  HIPCHK(hipMemcpyAsync(inst_ptr(c, c->psi, s), buf.data(), bytes, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipMemcpyAsync(inst_ptr(c, c->work, s), buf.data() + bytes, bytes, hipMemcpyHostToDevice,
+                        c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  in.step = h.step;
+  in.samples.clear();
+  in.have_rmse = false;
+  in.have_field = true;
+  return XPFC_OK;
+}
+
+extern "C" int xpfc_save_checkpoint(xpfc_ctx* c, const char* path) {
+  return xpfc_inst_save_checkpoint(c, 0, path);
+}
+extern "C" int xpfc_load_checkpoint(xpfc_ctx* c, const char* path) {
+  return xpfc_inst_load_checkpoint(c, 0, path);
+}

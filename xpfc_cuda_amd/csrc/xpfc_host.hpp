@@ -17,7 +17,8 @@ int fail(int code, const char* fmt, ...);
 // Reference initial-condition helpers shared with the context code.
 std::vector<double> linspace_ref(double x1, double x2, int N);
 void fix_mean(std::vector<double>& psi, double psi0);
-int read_vtk_file(const char* path, int* nx, int* ny, double* spacing, std::vector<double>* vals);
+// legacy VTK structured points: dims[3], spacing[3], values in file order (x fastest)
+int read_vtk_file(const char* path, int* dims, double* spacing, std::vector<double>* vals);
 double c2hat_host(double kmag, const xpfc_params& p);
 
 }  // namespace xpfc
@@ -80,6 +81,7 @@ struct xpfc_ctx {
   xpfc::InstArgs* d_ia = nullptr;  // [K + 1]; entry K: scratch for operator calls
   int* d_ids = nullptr;      // [K + 1] = 0 .. K
   int* d_active = nullptr;   // [K] active instance list
+  int* d_flags = nullptr;    // [K] divergence probe results
   int npx_cap = 0, npz_cap = 0;
   hipStream_t stream = nullptr;
   xpfc::KGeom g{};

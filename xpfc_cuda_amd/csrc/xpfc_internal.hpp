@@ -117,6 +117,9 @@ int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void
 int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s);
 int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s);
 int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s);
+// flag[s] = 1 if instance s's spectrum (at s * stride) is finite at two probe modes
+int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
+                 hipStream_t s);
 int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s);
 
 }  // namespace xpfc

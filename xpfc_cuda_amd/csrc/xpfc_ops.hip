@@ -125,6 +125,30 @@ static inline int grid_for(long long n, int bs) {
   return (int)g;
 }
 
+// Divergence probe: one workgroup; flag[s] = 1 when two modes of instance s
+// (k = (0,0,1) and the first Nyquist-plane mode) are finite.  A non-finite
+// value anywhere in real space reaches every mode at the next transform.
+template <typename T>
+__global__ void k_probe(const cpx<T>* __restrict__ psi, long long stride, long long iny, int K,
+                        int* __restrict__ flag) {
+  for (int s = threadIdx.x; s < K; s += blockDim.x) {
+    const cpx<T> a = psi[(long long)s * stride + 1];
+    const cpx<T> b = psi[(long long)s * stride + iny];
+    flag[s] = (isfinite(a.x) && isfinite(a.y) && isfinite(b.x) && isfinite(b.y)) ? 1 : 0;
+  }
+}
+
+int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
+                 hipStream_t s) {
+  if (prec == 1)
+    hipLaunchKernelGGL(k_probe<float>, dim3(1), dim3(256), 0, s, (const cpx<float>*)psi, stride,
+                       iny, K, flag);
+  else
+    hipLaunchKernelGGL(k_probe<double>, dim3(1), dim3(256), 0, s, (const cpx<double>*)psi, stride,
+                       iny, K, flag);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
 int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s) {
   if (prec == 1)
     hipLaunchKernelGGL(k_power<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, (float*)dst, n, p);

@@ -63,7 +63,7 @@ double c2hat_host(double kmag, const xpfc_params& p) {
   return r;
 }
 
-int read_vtk_file(const char* path, int* nx, int* ny, double* spacing, std::vector<double>* vals) {
+int read_vtk_file(const char* path, int* dims, double* spacing, std::vector<double>* vals) {
   FILE* f = std::fopen(path, "r");
   if (!f) return fail(XPFC_ERR_IO, "cannot open %s", path);
   char tt[80];
@@ -75,11 +75,11 @@ int read_vtk_file(const char* path, int* nx, int* ny, double* spacing, std::vect
   double dx = 0, dy = 0, dz = 0;
   ok = ok && std::fscanf(f, "SPACING %lf %lf %lf", &dx, &dy, &dz) == 3;
   for (int i = 0; i < 4 && ok; ++i) ok = std::fgets(tt, 80, f) != nullptr;  // rest, POINT_DATA, SCALARS, LOOKUP
-  if (!ok || mx <= 0 || my <= 0 || mz != 1) {
+  if (!ok || mx <= 0 || my <= 0 || mz <= 0 || (long long)mx * my * mz > (1LL << 31)) {
     std::fclose(f);
-    return fail(XPFC_ERR_PARSE, "%s: not a 2-D legacy VTK structured-points file", path);
+    return fail(XPFC_ERR_PARSE, "%s: not a legacy VTK structured-points file", path);
   }
-  vals->resize((size_t)mx * my);
+  vals->resize((size_t)mx * my * mz);
   for (size_t i = 0; i < vals->size(); ++i) {
     if (!std::fgets(tt, sizeof(tt), f) || std::sscanf(tt, "%lf", &(*vals)[i]) != 1) {
       std::fclose(f);
@@ -87,8 +87,9 @@ int read_vtk_file(const char* path, int* nx, int* ny, double* spacing, std::vect
     }
   }
   std::fclose(f);
-  *nx = mx;
-  *ny = my;
+  dims[0] = mx;
+  dims[1] = my;
+  dims[2] = mz;
   if (spacing) {
     spacing[0] = dx;
     spacing[1] = dy;
@@ -184,6 +185,7 @@ extern "C" int xpfc_read_input(const char* path, xpfc_params* out) {
     if (!std::strcmp(key, "dim")) std::sscanf(rest, "%d", &p.dim);
     else if (!std::strcmp(key, "Lz")) std::sscanf(rest, "%lf", &p.L[2]);
     else if (!std::strcmp(key, "Mz")) std::sscanf(rest, "%d", &p.n[2]);
+    else if (!std::strcmp(key, "restart_order")) std::sscanf(rest, "%d", &p.restart_order);
     else if (!std::strcmp(key, "precision")) {
       char v[32] = {0};
       std::sscanf(rest, "%31s", v);
@@ -230,6 +232,16 @@ extern "C" int xpfc_read_input(const char* path, xpfc_params* out) {
   p.L[1] = Ly;
   p.n[0] = Mx;
   p.n[1] = My;
+  if (p.iconfig == 10) {
+    // obtain_box_size (read_input.cu:124-157): L = round(M_t * spacing) per
+    // axis from the restart file in the working directory
+    int dims[3];
+    double sp[3];
+    std::vector<double> v;
+    int rc = xpfc::read_vtk_file("initial_density_field.vtk", dims, sp, &v);
+    if (rc) return rc;
+    for (int d = 0; d < 3; ++d) p.L[d] = (double)(int)std::lround(dims[d] * sp[d]);
+  }
   if (p.dim == 2) {
     p.n[2] = 1;
     p.L[2] = 1.0;
@@ -315,11 +327,11 @@ extern "C" int xpfc_randn(unsigned long long seed, unsigned long long first, lon
 // ---------------------------------------------------------------------------
 // VTK restart input (read_input.cu:161-241)
 // ---------------------------------------------------------------------------
-extern "C" int xpfc_read_vtk(const char* path, int* nx, int* ny, double* spacing, double* host_real,
+extern "C" int xpfc_read_vtk(const char* path, int* dims, double* spacing, double* host_real,
                              long long capacity) {
-  if (!path || !nx || !ny) return fail(XPFC_ERR_ARG, "null argument");
+  if (!path || !dims) return fail(XPFC_ERR_ARG, "null argument");
   std::vector<double> v;
-  int rc = xpfc::read_vtk_file(path, nx, ny, spacing, &v);
+  int rc = xpfc::read_vtk_file(path, dims, spacing, &v);
   if (rc) return rc;
   if (host_real) {
     if ((long long)v.size() > capacity) return fail(XPFC_ERR_ARG, "capacity too small");

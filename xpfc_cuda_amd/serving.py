@@ -174,15 +174,14 @@ class Service:
         for slot in finished:
             job = c.running.pop(slot)
             del c.remaining[slot]
-            try:
-                job.F_ref, job.f_mean = c.ctx.fe_calc(slot)
-                job.status = "done"
-            except XPFCError as e:
-                # the explicit nonlinear term can diverge for strongly
-                # crystallising parameters: the job fails, its slot is reused
-                if e.status != XPFC_ERR_NONFINITE:
-                    raise
-                job.status = "nonfinite"
+            if job.status != "nonfinite":
+                try:
+                    job.F_ref, job.f_mean = c.ctx.fe_calc(slot)
+                    job.status = "done"
+                except XPFCError as e:
+                    if e.status != XPFC_ERR_NONFINITE:
+                        raise
+                    job.status = "nonfinite"
             job.mean = c.ctx.avg_calc(slot)
             if self.keep_fields:
                 job.field = c.ctx.get_field(slot)
@@ -250,6 +249,14 @@ class Service:
                 if not c.ctx.query():
                     continue
                 c.busy = False
+                # the explicit nonlinear term diverges for strongly
+                # crystallising parameters: such a job stops at the end of
+                # the block it diverged in and its slot is reused
+                ok = c.ctx.probe_finite()
+                for slot, job in c.running.items():
+                    if not ok[slot]:
+                        job.status = "nonfinite"
+                        c.remaining[slot] = 0
                 self._complete(c)
                 worked = True
         self._admit()
