@@ -176,7 +176,7 @@ def test_variants_give_bitwise_identical_fields(shape, prec):
     op, lp = growth_case(shape, prec=prec, t_sample=0)
     psi0 = O.noise_ic(op, 0.01, 9)
     outs = []
-    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (17, 0, 257), (9, 0, 65)]:
+    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (16, 0, 257), (8, 0, 65)]:
         with Context(lp) as ctx:
             try:
                 ctx.set_variants(*v)
@@ -194,7 +194,7 @@ def test_autotune_picks_available_variant():
     _, lp = make((64, 64, 64), prec=XPFC_FP32_MIXED)
     with Context(lp) as ctx:
         v = ctx.autotune()
-        assert (v[0] & ~1) in (4, 8, 16) and v[1] in (4, 8, 16) and (v[2] & ~1) in (64, 128, 256, 512)
+        assert v[0] in (4, 8, 16) and v[1] in (4, 8, 16) and (v[2] & ~1) in (64, 128, 256, 512)
         assert ctx.variants() == v
 
 

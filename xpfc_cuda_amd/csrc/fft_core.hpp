@@ -8,7 +8,9 @@
 // Between stages the outputs go through LDS (one write + one read per point).
 //
 // Twiddles come from a host-built table tw[m] = exp(-2*pi*i*m/L), computed
-// in long double and stored in the working precision.
+// in long double and stored in the working precision; the kernels copy it into
+// LDS once per workgroup (stage_table), so a stage's twiddles are LDS reads
+// (mostly broadcasts) instead of vector-memory loads.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -181,6 +183,13 @@ __device__ __forceinline__ void fft_line(cpx<T> (&v)[E], int t, cpx<T>* lds, con
                                          const cpx<T>* __restrict__ tw) {
   static_assert(L % E == 0, "E must divide L");
   if constexpr (L > 1) fft_stages<T, L, E, DIR, 0>(v, t, lds, idx, tw);
+}
+
+// Copy an n-entry table into LDS with all NT threads of the workgroup; the
+// caller's next __syncthreads() publishes it.
+template <int NT, typename C>
+__device__ __forceinline__ void stage_table(C* dst, const C* __restrict__ src, int n) {
+  for (int i = threadIdx.x; i < n; i += NT) dst[i] = src[i];
 }
 
 // Row-major tile of TC columns: row r, column c -> r*(TC+PAD) + c.

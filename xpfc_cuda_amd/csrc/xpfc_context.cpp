@@ -77,6 +77,8 @@ void free_all(xpfc_ctx* c) {
                   c->pz, c->sums, c->d_ia, c->d_ids, c->d_active, c->d_flags};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
+  for (XInst& in : c->inst)
+    if (in.c2tab) (void)hipFree(in.c2tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
 }
 
@@ -112,6 +114,24 @@ int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
   }
   ph.kcut2 = kc * kc;
   in->c2zero = c2hat_host(0.0, p);
+  if (c->g.cubic) {
+    // C2 table over m = |k|^2 / dk^2 up to the cutoff (or the largest m on the grid)
+    const double mmax = (double)(c->n0 / 2) * (c->n0 / 2) + (double)(c->n1 / 2) * (c->n1 / 2) +
+                        (double)c->H * c->H;
+    const int mcut = (int)std::min(std::ceil(ph.kcut2 / c->g.dksq) + 1.0, mmax + 1.0);
+    if (mcut > in->c2cap) {
+      HIPCHK(hipStreamSynchronize(c->stream));  // queued steps may still read the old table
+      if (in->c2tab) (void)hipFree(in->c2tab);
+      in->c2tab = nullptr;
+      in->c2cap = 0;
+      if (hipMalloc((void**)&in->c2tab, (size_t)mcut * sizeof(double)) != hipSuccess)
+        return fail(XPFC_ERR_NOMEM, "C2 table allocation failed");
+      in->c2cap = mcut;
+    }
+    ph.mcut = mcut;
+    ph.c2tab = in->c2tab;
+    LCHK(launch_c2table(in->c2tab, mcut, c->g.dksq, ph, c->stream));
+  }
   in->ia.za.a = p.a;
   in->ia.za.b = p.b;
   in->ia.za.scale = 1.0 / (double)c->N;
@@ -250,6 +270,8 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   c->g.dk0 = 2.0 * M_PI / L0;
   c->g.dk1 = nd == 3 ? 2.0 * M_PI / L1 : 0.0;
   c->g.dk2 = 2.0 * M_PI / L2;
+  c->g.cubic = (c->g.dk0 == c->g.dk2 && (nd == 2 || c->g.dk1 == c->g.dk0)) ? 1 : 0;
+  c->g.dksq = c->g.dk0 * c->g.dk0;
 
   c->inst.resize(K);
   std::vector<InstArgs> ia(K + 1);
@@ -374,13 +396,12 @@ namespace {
 
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
   if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->stream));
-  if (do_x)
-    LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, -1, c->tw0, bt, ny, c->stream));
+  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, bt, ny, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
-  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x & ~XOCC_HINT, +1, c->tw0, bt, ny, c->stream));
+  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, bt, ny, c->stream));
   if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->stream));
   return XPFC_OK;
 }
@@ -825,8 +846,8 @@ int autotune(xpfc_ctx* c) {
     rc = fail(XPFC_ERR_DEVICE, "autotune scratch clear failed");
   const int reps = c->N * ny >= (1LL << 24) ? 4 : 16;
   int best[3] = {c->tc_x, c->tc_y, c->ntz};
-  // axis-0: tile width (+XOCC_HINT); axis-1: tile width; real space: block (+ZFREE_REGS)
-  const std::vector<int> xs{4, 8, 16, 4 | XOCC_HINT, 8 | XOCC_HINT, 16 | XOCC_HINT};
+  // axis-0 and axis-1: tile width; real space: block (+ZFREE_REGS)
+  const std::vector<int> xs{4, 8, 16};
   const std::vector<int> ys{4, 8, 16};
   const std::vector<int> zs{64, 128, 256, 512, 64 | ZFREE_REGS, 128 | ZFREE_REGS,
                             256 | ZFREE_REGS, 512 | ZFREE_REGS};

@@ -20,6 +20,11 @@ __device__ __forceinline__ double c2hat(double kmag, const Phys& ph) {
   return r;
 }
 
+// C2 at k^2 = dksq * m on a cubic grid, from the instance's table
+__device__ __forceinline__ double c2_lookup(int m, const Phys& ph) {
+  return m < ph.mcut ? ph.c2tab[m] : 0.0;
+}
+
 // FFT_freq.cu:17-24 index -> signed wavenumber index (Nyquist positive)
 __device__ __forceinline__ int signed_index(int i, int n) { return i <= n / 2 ? i : i - n; }
 

@@ -51,6 +51,8 @@ struct XInst {
   std::vector<xpfc_sample> samples;
   double last_rmse = 0.0;
   bool have_rmse = false;
+  double* c2tab = nullptr;  // device C2 table (square / cubic grids)
+  int c2cap = 0;            // its capacity in entries
 };
 
 // One sampled step awaiting read-back: its row in the sums buffer and the

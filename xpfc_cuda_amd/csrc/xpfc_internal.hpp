@@ -27,6 +27,10 @@ struct Segs {
 struct KGeom {
   int n0, n1, H;
   double dk0, dk1, dk2;  // 2*pi/L per axis
+  // Equal spacing on every axis (square / cubic boxes): k^2 = dksq * m with the
+  // integer m = i^2 + j^2 + l^2, and C2 comes from a per-instance table over m.
+  int cubic;
+  double dksq;
 };
 
 // Physics constants of the semi-implicit update (all FP64).
@@ -41,6 +45,9 @@ struct Phys {
   int npk;
   int dcf_max;       // 0 = sum of peaks, 1 = max over peaks
   double pk[4], inv2a2[4], amp[4];
+  // cubic geometry: c2tab[m] = C2(sqrt(dksq * m)) for m < mcut, 0 beyond
+  int mcut;
+  const double* c2tab;
 };
 
 // Real-space pass constants.
@@ -70,11 +77,9 @@ struct Batch {
   long long pstride;
   long long pzstride;
 };
-// Register-allocation variant flags (the arithmetic is identical):
-// or-ed into the real-space workgroup size: no occupancy hint;
-// or-ed into the axis-0 tile width: budget for two 1024-thread workgroups/CU.
+// Register-allocation variant flag (the arithmetic is identical), or-ed into
+// the real-space workgroup size: no occupancy hint.
 constexpr int ZFREE_REGS = 1;
-constexpr int XOCC_HINT = 1;
 
 // Per-workgroup FP64 partial sums.  X pass: {sum w c2 |psi_k|^2, sum w |R_k|^2}.
 // Z pass: {sum psi, sum psi^2, sum psi^3, sum psi^4}.
@@ -103,6 +108,8 @@ int launch_finalize(const double* px, int nx, const double* pz, int nz, const Ba
                     double* out, hipStream_t s);
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
                        const Batch& bt, double* partial, int* nblocks, hipStream_t s);
+// tab[m] = C2(sqrt(dksq * m)), m < mcut, for the physics `ph`
+int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s);
 
 // operator-level kernels on natural layouts (real [N], spectrum [n0][n1][n2/2+1])
 int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s);

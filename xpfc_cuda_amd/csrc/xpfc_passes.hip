@@ -31,9 +31,10 @@ constexpr int PADT = 1;  // LDS padding (complex elements) per tile row
 
 template <int L> constexpr int elems_per_thread() { return L < 8 ? L : 8; }
 template <int L, int TC> constexpr int tile_threads() { return TC * (L / elems_per_thread<L>()); }
+// LDS: the tile plus the L-entry twiddle table
 template <typename T, int L, int TC> constexpr bool tile_ok() {
   return tile_threads<L, TC>() <= 1024 &&
-         (long long)L * (TC + PADT) * (long long)sizeof(cpx<T>) <= 144 * 1024;
+         (long long)L * (TC + PADT + 1) * (long long)sizeof(cpx<T>) <= 152 * 1024;
 }
 
 // XCD-aware block order: the dispatcher deals consecutive workgroups to the 8
@@ -74,6 +75,7 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   __shared__ C lds[L * (TC + PADT)];
+  __shared__ C twl[L];
   buf += (long long)bt.slots[blockIdx.y] * bt.stride;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
@@ -82,7 +84,8 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = loc.act ? p[(t + m * (L / E)) * rs] : mk<C>(0, 0);
-  fft_line<T, L, E, DIR>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+  stage_table<tile_threads<L, TC>()>(twl, tw, L);  // read after the first exchange's barrier
+  fft_line<T, L, E, DIR>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
   if (loc.act) {
 #pragma unroll
     for (int m = 0; m < E; ++m) p[(t + m * (L / E)) * rs] = v[m];
@@ -92,16 +95,19 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
 // ---------------------------------------------------------------------------
 // fused axis-0 pass: FFT(N) along axis 0, semi-implicit update, inverse FFT
 // ---------------------------------------------------------------------------
-// OCC: register budget for two 1024-thread workgroups per CU (FP32: 64 VGPRs,
-// FP64: 128), at the price of spills; a register-allocation variant only.
-template <typename T, int L, int TC, bool SAMPLE, bool OCC>
-__global__ __launch_bounds__((tile_threads<L, TC>()), (OCC ? (sizeof(T) == 4 ? 8 : 4) : 1))
+// LUT: square / cubic grid, k^2 = dksq * m from the integer m = i^2 + j^2 + l^2
+// and C2 from the instance's table over m (no FP64 exp in the kernel, which
+// keeps the register footprint of the update small); otherwise k^2 and C2 are
+// evaluated from the per-axis wavenumbers.
+template <typename T, int L, int TC, bool SAMPLE, bool LUT>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
 void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
              const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   constexpr int NT = tile_threads<L, TC>();
   __shared__ C lds[L * (TC + PADT)];
+  __shared__ C twl[L];
   const int slot = bt.slots[blockIdx.y];
   work += (long long)slot * bt.stride;
   psi += (long long)slot * bt.stride;
@@ -117,13 +123,16 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
-  fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+  stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
+  fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
 
   // wavenumber of this column
   int kb, kz;
   if (loc.sg->kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
   else { kb = loc.col; kz = g.H; }
-  const double ky = g.dk1 * (double)signed_index(kb, g.n1);
+  const int jb = signed_index(kb, g.n1);
+  const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
+  const double ky = g.dk1 * (double)jb;
   const double kzv = g.dk2 * (double)kz;
   const double kyz2 = ky * ky + kzv * kzv;
   const double wgt = act ? ((loc.sg->kind == 0 && kz == 0) || loc.sg->kind == 1 ? 1.0 : 2.0) : 0.0;
@@ -141,9 +150,17 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
     const int ka = t + m * (L / E);
     const C old = nxt;
     if (m + 1 < E) nxt = act ? pp[(ka + L / E) * rs] : mk<C>(0, 0);  // prefetch
-    const double kx = g.dk0 * (double)signed_index(ka, g.n0);
-    const double k2 = kx * kx + kyz2;
-    const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    const int ia = signed_index(ka, g.n0);
+    double k2, c2;
+    if constexpr (LUT) {
+      const int mm = ia * ia + myz;
+      k2 = g.dksq * (double)mm;
+      c2 = c2_lookup(mm, ph);
+    } else {
+      const double kx = g.dk0 * (double)ia;
+      k2 = kx * kx + kyz2;
+      c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    }
     const double a2 = k2 * ph.dtM;
     const double den = 1.0 + a2 * ph.one_m_eps - a2 * c2;  // euler_step.cu:54
     double inv = __builtin_amdgcn_rcp(den);                 // v_rcp_f64 + 2 Newton steps
@@ -169,7 +186,7 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   }
   __builtin_amdgcn_sched_barrier(0);
 
-  fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, tw);
+  fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
   if (act) {
 #pragma unroll
     for (int m = 0; m < E; ++m) pw[(t + m * (L / E)) * rs] = v[m];
@@ -187,7 +204,8 @@ template <typename T, int H> constexpr int z_elems() {
 }
 template <typename T, int H> constexpr int z_tpl() { return H / z_elems<T, H>(); }  // threads per line
 template <typename T, int H, int NTZ> constexpr bool z_ok() {
-  return NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0;
+  return NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0 &&
+         ((long long)(NTZ / z_tpl<T, H>()) * (H + 1) + 2 * H) * (long long)sizeof(cpx<T>) <= 152 * 1024;
 }
 // LDS line of H rows plus the X[H] slot.  Rows are XOR-swizzled within each
 // block of E: a Stockham stage writes rows E*j + q from lanes j, and the swizzle
@@ -227,6 +245,10 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   constexpr int TPL = z_tpl<T, H>();
   constexpr int G = NTZ / TPL;
   __shared__ C lds[G * z_len<H>()];
+  __shared__ C twl[H], twpl[H];  // the H-point FFT twiddles and the half-spectrum twiddles
+  stage_table<NTZ>(twl, tw, H);
+  stage_table<NTZ>(twpl, twp, H);
+  __syncthreads();
   const int tid = threadIdx.x;
   const int g = tid / TPL, t = tid - g * TPL;
   const long long line = (long long)blockIdx.x * G + g;
@@ -246,10 +268,10 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
       if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist bins are real (C2R)
       const C e = mk<C>(xk.x + xm.x, xk.y - xm.y);
       const C d = mk<C>(xk.x - xm.x, xk.y + xm.y);
-      const C o = cmul(d, cconj(twp[k]));
+      const C o = cmul(d, cconj(twpl[k]));
       v[m] = mk<C>((e.x - o.y) * s, (e.y + o.x) * s);
     }
-    fft_line<T, H, E, +1>(v, t, lds, idx, tw);
+    fft_line<T, H, E, +1>(v, t, lds, idx, twl);
     if constexpr (MODE == Z_C2R) {
       C* rp = reinterpret_cast<C*>(out) + line * H;  // real pairs
 #pragma unroll
@@ -302,7 +324,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   }
 
   // forward H-point transform and the real-to-half-spectrum post twiddle
-  fft_line<T, H, E, -1>(v, t, lds, idx, tw);
+  fft_line<T, H, E, -1>(v, t, lds, idx, twl);
   __syncthreads();
 #pragma unroll
   for (int m = 0; m < E; ++m) lds[idx(t + m * TPL)] = v[m];
@@ -321,7 +343,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
         op[0] = mk<C>(e.x + o.x, 0);
         out_ny[line] = mk<C>(e.x - o.x, 0);
       } else {
-        op[k] = cadd(e, cmul(twp[k], o));
+        op[k] = cadd(e, cmul(twpl[k], o));
       }
     }
   }
@@ -380,10 +402,15 @@ __global__ __launch_bounds__(256) void k_kspace_sums(const cpx<T>* __restrict__ 
       kz = g.H;
       w = 1.0;
     }
-    const double kx = g.dk0 * signed_index(ka, g.n0), ky = g.dk1 * signed_index(kb, g.n1),
-                 kzv = g.dk2 * kz;
-    const double k2 = kx * kx + ky * ky + kzv * kzv;
-    const double c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    const int ia = signed_index(ka, g.n0), jb = signed_index(kb, g.n1);
+    double c2;
+    if (g.cubic) {  // the same C2 values as the axis-0 pass
+      c2 = c2_lookup(ia * ia + jb * jb + kz * kz, ph);
+    } else {
+      const double kx = g.dk0 * ia, ky = g.dk1 * jb, kzv = g.dk2 * kz;
+      const double k2 = kx * kx + ky * ky + kzv * kzv;
+      c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+    }
     const cpx<T> v = psi[e];
     acc[0] += w * c2 * ((double)v.x * v.x + (double)v.y * v.y);
   }
@@ -463,7 +490,6 @@ int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir
 
 template <typename T>
 static int blocks_T(int L, int tc, const Segs& segs0) {
-  tc &= ~XOCC_HINT;
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
@@ -479,30 +505,32 @@ int strided_blocks(int prec, int L, int tc, const Segs& segs) {
   return prec == 1 ? blocks_T<float>(L, tc, segs) : blocks_T<double>(L, tc, segs);
 }
 
-// tc: tile width, plus XOCC_HINT for the two-workgroups-per-CU register budget
-// (sampled steps always use the unconstrained build)
+// tc: tile width; the square / cubic table path is taken when g.cubic is set
 template <typename T>
 static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, const void* tw,
                    const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
                    hipStream_t s) {
   int nb = 0;
-  const bool occ = (tc & XOCC_HINT) != 0;
-  tc &= ~XOCC_HINT;
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       nb = sg.tiles0 + sg.tiles1;
       if (ny == 0) return nb;
       const dim3 grid(nb, ny), block(tile_threads<LL, TT>());
-      if (sample)
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, true, false>), grid, block, 0, s, (cpx<T>*)work,
-                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
-      else if (occ)
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, true>), grid, block, 0, s, (cpx<T>*)work,
-                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
-      else
-        hipLaunchKernelGGL((k_xstep<T, LL, TT, false, false>), grid, block, 0, s, (cpx<T>*)work,
-                           (cpx<T>*)psi, sg, (const cpx<T>*)tw, g, bt, partial);
+      cpx<T>* w = (cpx<T>*)work;
+      cpx<T>* p = (cpx<T>*)psi;
+      const cpx<T>* t = (const cpx<T>*)tw;
+      if (g.cubic) {
+        if (sample)
+          hipLaunchKernelGGL((k_xstep<T, LL, TT, true, true>), grid, block, 0, s, w, p, sg, t, g, bt, partial);
+        else
+          hipLaunchKernelGGL((k_xstep<T, LL, TT, false, true>), grid, block, 0, s, w, p, sg, t, g, bt, partial);
+      } else {
+        if (sample)
+          hipLaunchKernelGGL((k_xstep<T, LL, TT, true, false>), grid, block, 0, s, w, p, sg, t, g, bt, partial);
+        else
+          hipLaunchKernelGGL((k_xstep<T, LL, TT, false, false>), grid, block, 0, s, w, p, sg, t, g, bt, partial);
+      }
     } else {
       return -3;
     }
@@ -609,6 +637,19 @@ int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny
   else
     hipLaunchKernelGGL(k_kspace_sums<double>, dim3(nb), dim3(256), 0, s, (const cpx<double>*)psi,
                        nmain, nny, g, bt, partial);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+// C2 table of one instance over the integer m = |k|^2 / dksq
+__global__ __launch_bounds__(256) void k_c2table(double* __restrict__ tab, int mcut, double dksq,
+                                                 Phys ph) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m < mcut) tab[m] = c2hat(sqrt(dksq * (double)m), ph);
+}
+
+int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s) {
+  if (mcut <= 0) return 0;
+  hipLaunchKernelGGL(k_c2table, dim3((mcut + 255) / 256), dim3(256), 0, s, tab, mcut, dksq, ph);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
