@@ -63,7 +63,8 @@ def gpu_run(n, prec, steps, warmup, seed=2):
         wall = time.perf_counter() - t0
         passes = ctx.time_passes(max(5, min(steps, 50)))
         samples = ctx.samples()
-    return ms, wall, passes, samples
+        variants = ctx.variants()
+    return ms, wall, passes, samples, variants
 
 
 def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
@@ -220,7 +221,7 @@ def main():
               flush=True)
         return
     n = args.n
-    ms, wall, passes, samples = gpu_run(n, 1, args.steps, args.warmup)
+    ms, wall, passes, samples, variants = gpu_run(n, 1, args.steps, args.warmup)
     ms_per_step = ms / args.steps
     value = 1000.0 / ms_per_step
     S = spectrum_bytes(n, 1)
@@ -257,16 +258,21 @@ def main():
     }
     if not args.no_fp64:
         fsteps = max(10, args.steps // 4)
-        ms64, _, passes64, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
+        ms64, _, passes64, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
         v64 = 1000.0 / (ms64 / fsteps)
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
         out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
+    out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1],
+                       "real_space_block": variants[2]}
     tp = os.path.join(ROOT, "profiles", "traffic_512cube_fp32.json")
     if n == 512 and os.path.exists(tp):
-        # HBM bytes per k_xstep launch from the committed FETCH_SIZE/WRITE_SIZE runs
+        # HBM bytes per k_xstep launch of the variant in use, from the committed
+        # FETCH_SIZE / WRITE_SIZE runs
         with open(tp) as f:
-            out["roofline"]["traffic"] = json.load(f)["k_xstep"]["traffic_bytes"]
+            tr = json.load(f)["k_xstep"].get(str(variants[0]))
+        if tr:
+            out["roofline"]["traffic"] = tr["traffic_bytes"]
     if not args.no_batch:
         out["batched"] = batched_run()
     if args.serve > 0:
