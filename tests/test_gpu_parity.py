@@ -218,3 +218,24 @@ def test_graph_replays_match_eager(prec):
     a = np.array(fields[0][1], dtype=float)
     b = np.array(fields[1][1], dtype=float)
     assert a.shape == b.shape and np.array_equal(a, b, equal_nan=True)
+
+
+@pytest.mark.parametrize("shape", [(64, 32), (16, 8, 32)])
+@pytest.mark.parametrize("prec", PRECS)
+def test_device_noise_matches_host_draws(shape, prec):
+    """initialize_noise builds psi0 + A*randn on the device; the same field
+    built on the host from xpfc_randn (mean re-fixed) must agree to rounding."""
+    import xpfc_cuda_amd as X
+    _, lp = make(shape, prec=prec, psi0=0.04)
+    n = int(np.prod(shape))
+    z = X.randn(77, n).reshape(shape)
+    host = 0.04 + 0.01 * z
+    host += 0.04 - host.mean()
+    with Context(lp) as c:
+        c.initialize_noise(0.01, 77)
+        dev = c.get_field()
+        assert c.avg_calc() == 0.04
+        c.set_field(host)
+        ref = c.get_field()
+    tol = 1e-16 if prec == XPFC_FP64 else 4e-9
+    assert np.max(np.abs(dev - ref)) <= tol

@@ -162,6 +162,9 @@ void* inst_ptr(const xpfc_ctx* c, void* base, int s) {
 // Reads the sums of pending sampled steps back into the sample logs (defined
 // with the step driver).
 int drain(xpfc_ctx* c);
+// Builds instance s's spectra from the fluctuation held in its aux region
+// (defined with the field I/O).
+int field_from_aux(xpfc_ctx* c, int s);
 
 int check_inst(const xpfc_ctx* c, int s) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
@@ -454,6 +457,14 @@ extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
   c->inst[s].mean = mean;
   if ((rc = upload_inst(c, s))) return rc;
   if ((rc = upload_fluct(c, s, host, mean))) return rc;
+  return field_from_aux(c, s);
+}
+
+namespace {
+// instance s's aux region holds the fluctuation psi - mean as reals: build the
+// state spectrum and the nonlinear-term spectrum from it
+int field_from_aux(xpfc_ctx* c, int s) {
+  int rc = XPFC_OK;
   const long long nl = (long long)c->n0 * c->n1;
   const Batch bt = bt_one(c, s);
   // instance s's constants, pointers already at instance s's regions
@@ -474,6 +485,7 @@ extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
   c->inst[s].have_field = true;
   return XPFC_OK;
 }
+}  // namespace
 
 extern "C" int xpfc_inst_get_field(xpfc_ctx* c, int s, double* host) {
   int rc = check_inst(c, s);
@@ -578,15 +590,28 @@ extern "C" int xpfc_inst_initialize_vtk(xpfc_ctx* c, int s, const char* path) {
   return xpfc_inst_set_field(c, s, psi.data());
 }
 
+// psi = psi0 + amplitude * randn(seed, cell) with its mean fixed to psi0, i.e.
+// psi - psi0 = amplitude * (z - mean z): generated on the device with the
+// draws of xpfc_randn, the mean of z summed in a fixed order
 extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, unsigned long long seed) {
   int rc = check_inst(c, s);
   if (rc) return rc;
-  std::vector<double> psi((size_t)c->N);
-  if ((rc = xpfc_randn(seed, 0, c->N, psi.data()))) return rc;
-  const double psi0 = c->inst[s].p.psi0;
-  for (double& v : psi) v = psi0 + amplitude * v;
-  fix_mean(psi, psi0);
-  return xpfc_inst_set_field(c, s, psi.data());
+  if (!std::isfinite(amplitude)) return fail(XPFC_ERR_ARG, "non-finite amplitude");
+  if ((rc = drain(c)) != XPFC_OK) return rc;  // pending records use the old mean
+  double* part = c->px + (size_t)s * c->npx_cap * XPART;  // >= 4096 doubles
+  int nb = 0;
+  LCHK(launch_noise_sum(c->N, seed, part, &nb, c->stream));
+  std::vector<double> h(nb);
+  HIPCHK(hipMemcpyAsync(h.data(), part, nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  double sz = 0.0;
+  for (double v : h) sz += v;
+  const double mean_z = sz / (double)c->N;
+  if (!std::isfinite(mean_z)) return fail(XPFC_ERR_NONFINITE, "noise draws not finite");
+  c->inst[s].mean = c->inst[s].p.psi0;
+  if ((rc = upload_inst(c, s))) return rc;
+  LCHK(launch_noise_fill(c->prec, inst_ptr(c, c->aux, s), c->N, seed, amplitude, mean_z, c->stream));
+  return field_from_aux(c, s);
 }
 
 extern "C" int xpfc_initialize(xpfc_ctx* c) { return xpfc_inst_initialize(c, 0); }

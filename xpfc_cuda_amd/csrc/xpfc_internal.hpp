@@ -124,6 +124,11 @@ int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void
 int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s);
 int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s);
 int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s);
+// noise initial condition: per-workgroup sums of the draws of cells 0..n-1
+// (nblocks returned, <= 4096), then out[c] = amp * (z_c - mean_z) as reals
+int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s);
+int launch_noise_fill(int prec, void* out, long long n, unsigned long long seed, double amp,
+                      double mean_z, hipStream_t s);
 // flag[s] = 1 if instance s's spectrum (at s * stride) is finite at two probe modes
 int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
                  hipStream_t s);

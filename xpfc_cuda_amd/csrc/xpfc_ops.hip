@@ -3,6 +3,8 @@
 // parity of each operator.  The time step itself does not use them.
 #include "xpfc_device.hpp"
 
+#include <algorithm>
+
 namespace xpfc {
 
 // ---------------------------------------------------------------------------
@@ -123,6 +125,69 @@ static inline int grid_for(long long n, int bs) {
   if (g > 65536) g = 65536;
   if (g < 1) g = 1;
   return (int)g;
+}
+
+// ---------------------------------------------------------------------------
+// noise initial condition on the device: the draws of xpfc_randn (Philox4x32-10
+// with counter (cell_lo, cell_hi, 0, 0) and key (seed_lo, seed_hi), Box-Muller
+// in FP64), so a job's field is built without a host round trip
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double randn_cell(unsigned long long cell, uint32_t k0, uint32_t k1) {
+  uint32_t c0 = (uint32_t)cell, c1 = (uint32_t)(cell >> 32), c2 = 0u, c3 = 0u;
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) {
+      k0 += 0x9E3779B9u;
+      k1 += 0xBB67AE85u;
+    }
+    const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+    const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+    const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+    c0 = n0;
+    c1 = lo1;
+    c2 = n2;
+    c3 = lo0;
+  }
+  const double s32 = 1.0 / 4294967296.0;
+  const double u0 = ((double)c0 + 0.5) * s32, u1 = ((double)c1 + 0.5) * s32;
+  return sqrt(-2.0 * log(u0)) * cos(6.283185307179586 * u1);
+}
+
+// per-workgroup FP64 sums of the draws (fixed order)
+__global__ __launch_bounds__(256) void k_noise_sum(long long n, uint32_t k0, uint32_t k1,
+                                                   double* __restrict__ partial) {
+  double a[1] = {0.0};
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
+    a[0] += randn_cell((unsigned long long)c, k0, k1);
+  block_sum<256, 1>(a, partial + blockIdx.x);
+}
+
+// out[c] = amp * (z_c - mean_z): the fluctuation about the mean psi0
+template <typename T>
+__global__ __launch_bounds__(256) void k_noise_fill(T* __restrict__ out, long long n, uint32_t k0,
+                                                    uint32_t k1, double amp, double mean_z) {
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
+    out[c] = (T)(amp * (randn_cell((unsigned long long)c, k0, k1) - mean_z));
+}
+
+int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s) {
+  const int nb = (int)std::min<long long>(4096, std::max<long long>(1, (n + 255) / 256));
+  *nblocks = nb;
+  hipLaunchKernelGGL(k_noise_sum, dim3(nb), dim3(256), 0, s, n, (uint32_t)seed, (uint32_t)(seed >> 32),
+                     partial);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
+}
+
+int launch_noise_fill(int prec, void* out, long long n, unsigned long long seed, double amp,
+                      double mean_z, hipStream_t s) {
+  const int nb = (int)std::min<long long>(8192, std::max<long long>(1, (n + 255) / 256));
+  if (prec == 1)
+    hipLaunchKernelGGL(k_noise_fill<float>, dim3(nb), dim3(256), 0, s, (float*)out, n, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), amp, mean_z);
+  else
+    hipLaunchKernelGGL(k_noise_fill<double>, dim3(nb), dim3(256), 0, s, (double*)out, n, (uint32_t)seed,
+                       (uint32_t)(seed >> 32), amp, mean_z);
+  return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
 // Divergence probe: one workgroup; flag[s] = 1 when two modes of instance s
