@@ -64,7 +64,16 @@ def gpu_run(n, prec, steps, warmup, seed=2):
         passes = ctx.time_passes(max(5, min(steps, 50)))
         samples = ctx.samples()
         variants = ctx.variants()
-    return ms, wall, passes, samples, variants
+        # host-buffer boundary (outside the timed region): field in and out
+        io = None
+        if prec == 1:
+            t0 = time.perf_counter()
+            field = ctx.get_field()
+            t1 = time.perf_counter()
+            ctx.set_field(field)
+            t2 = time.perf_counter()
+            io = {"get_field_ms": round((t1 - t0) * 1e3, 2), "set_field_ms": round((t2 - t1) * 1e3, 2)}
+    return ms, wall, passes, samples, variants, io
 
 
 def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
@@ -221,7 +230,7 @@ def main():
               flush=True)
         return
     n = args.n
-    ms, wall, passes, samples, variants = gpu_run(n, 1, args.steps, args.warmup)
+    ms, wall, passes, samples, variants, io = gpu_run(n, 1, args.steps, args.warmup)
     ms_per_step = ms / args.steps
     value = 1000.0 / ms_per_step
     S = spectrum_bytes(n, 1)
@@ -258,13 +267,18 @@ def main():
     }
     if not args.no_fp64:
         fsteps = max(10, args.steps // 4)
-        ms64, _, passes64, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
+        ms64, _, passes64, _, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
         v64 = 1000.0 / (ms64 / fsteps)
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
         out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
     out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1],
                        "real_space_block": variants[2]}
+    if io:
+        # a 1000-step job that takes its field from host memory and returns it
+        io_ms = io["get_field_ms"] + io["set_field_ms"]
+        io["steps_per_s_1000_step_job_incl_host_io"] = round(1000.0 / (1000 * ms_per_step + io_ms) * 1e3, 2)
+        out["host_io"] = io
     tp = os.path.join(ROOT, "profiles", "traffic_512cube_fp32.json")
     if n == 512 and os.path.exists(tp):
         # HBM bytes per k_xstep launch of the variant in use, from the committed

@@ -239,3 +239,31 @@ def test_device_noise_matches_host_draws(shape, prec):
         ref = c.get_field()
     tol = 1e-16 if prec == XPFC_FP64 else 4e-9
     assert np.max(np.abs(dev - ref)) <= tol
+
+
+@pytest.mark.parametrize("sigma", [0.1])
+def test_fcc_64cube_fp32_1000_steps(sigma):
+    """The 512^3 workload's physics (fcc two-peak kernel, max envelope,
+    dx = 0.125, dt = 0.5, sigma = 0.1: a liquid whose noise decays) at 64^3,
+    FP32-mixed vs the FP64 oracle after 1000 steps: max|dn| <= 1e-5, FE
+    relative error <= 1e-7.  (The crystallising sigma = 0.07 diverges under the
+    explicit nonlinear term within 250 steps, in the oracle too.)"""
+    import math
+    _flush("fcc 64^3 sigma=%g" % sigma)
+    k1 = 2.0 * math.pi
+    peaks = [(k1, 1.0, 1.0), (k1 * 2.0 / math.sqrt(3.0), 1.0, 1.0)]
+    op, lp = make((64, 64, 64), sigma=sigma, psi0=0.05, peaks=peaks, prec=XPFC_FP32_MIXED,
+                  t_sample=250)
+    psi0 = O.noise_ic(op, 0.01, 2)
+    ref, rs = O.run(psi0, op, 1000, sample=True, workers=8)
+    with Context(lp) as ctx:
+        ctx.set_field(psi0)
+        ctx.step(1000)
+        got = ctx.get_field()
+        gs = ctx.samples()
+    assert np.all(np.isfinite(ref))
+    assert np.max(np.abs(got - ref)) <= 1e-5
+    gfe = {s.step: s.F_ref for s in gs if s.has_fe}
+    for s in rs:
+        if s.kind == "FE":
+            assert abs(gfe[s.step] - s.value) <= 1e-7 * abs(s.value), (s.step, gfe[s.step], s.value)
