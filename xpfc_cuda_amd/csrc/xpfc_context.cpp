@@ -18,6 +18,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 using namespace xpfc;
@@ -29,6 +30,57 @@ constexpr int DIAG_ROW = SUM_ROWS;  // extra row for on-demand diagnostics
 constexpr int GRAPH_BLOCK = 50;  // steps per captured step block
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+// Runs f(i0, i1) over [0, n) on up to 16 host threads (field conversions at
+// the host boundary).
+template <class F>
+void host_parallel(long long n, F f, long long grain = 1LL << 18) {
+  const int nt = (int)std::min<long long>({16LL, std::max(1LL, n / grain),
+                                           (long long)std::max(1u, std::thread::hardware_concurrency())});
+  if (nt <= 1) {
+    f(0LL, n);
+    return;
+  }
+  std::vector<std::thread> th;
+  const long long per = (n + nt - 1) / nt;
+  for (int t = 0; t < nt; ++t) {
+    const long long i0 = t * per, i1 = std::min(n, i0 + per);
+    if (i0 < i1) th.emplace_back(f, i0, i1);
+  }
+  for (auto& x : th) x.join();
+}
+
+// FP64 sum in a fixed order: 256 contiguous blocks summed sequentially, then
+// the block sums in order (independent of the thread count).
+double fixed_order_sum(const double* v, long long n) {
+  constexpr int NB = 256;
+  double part[NB];
+  // one thread per >= 2^18 elements, as for the conversions
+  const long long grain = std::max(1LL, (long long)NB * (1LL << 18) / std::max(1LL, n));
+  host_parallel(NB, [&](long long b0, long long b1) {
+    for (long long b = b0; b < b1; ++b) {
+      const long long i0 = n * b / NB, i1 = n * (b + 1) / NB;
+      double s = 0.0;
+      for (long long i = i0; i < i1; ++i) s += v[i];
+      part[b] = s;
+    }
+  }, grain);
+  double s = 0.0;
+  for (int b = 0; b < NB; ++b) s += part[b];
+  return s;
+}
+
+// Pinned host staging buffer of one real field in the working precision
+// (allocated on first use, kept until the context is destroyed).
+void* staging(xpfc_ctx* c) {
+  if (!c->stage) {
+    if (hipHostMalloc(&c->stage, (size_t)c->N * c->rsize, hipHostMallocDefault) != hipSuccess) {
+      c->stage = nullptr;
+      return nullptr;
+    }
+  }
+  return c->stage;
+}
 
 template <typename T>
 std::vector<T> twiddle_table(int L, int denom) {
@@ -80,6 +132,7 @@ void free_all(xpfc_ctx* c) {
   for (XInst& in : c->inst)
     if (in.c2tab) (void)hipFree(in.c2tab);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stage) (void)hipHostFree(c->stage);
 }
 
 bool same_grid(const xpfc_params& a, const xpfc_params& b) {
@@ -426,19 +479,24 @@ int psi_to_real(xpfc_ctx* c, int s, bool sums) {
 
 // fluctuation (host FP64) -> instance s's aux region (working precision)
 int upload_fluct(xpfc_ctx* c, int s, const double* host, double mean) {
-  const size_t n = (size_t)c->N;
+  const long long n = c->N;
   void* dst = inst_ptr(c, c->aux, s);
+  void* st = staging(c);
+  if (!st) return fail(XPFC_ERR_NOMEM, "pinned staging allocation failed");
+  HIPCHK(hipStreamSynchronize(c->stream));  // the staging buffer may still feed a copy
   if (c->prec == 1) {
-    std::vector<float> f(n);
-    for (size_t i = 0; i < n; ++i) f[i] = (float)(host[i] - mean);
-    HIPCHK(hipMemcpyAsync(dst, f.data(), n * 4, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    float* f = (float*)st;
+    host_parallel(n, [&](long long i0, long long i1) {
+      for (long long i = i0; i < i1; ++i) f[i] = (float)(host[i] - mean);
+    });
   } else {
-    std::vector<double> f(n);
-    for (size_t i = 0; i < n; ++i) f[i] = host[i] - mean;
-    HIPCHK(hipMemcpyAsync(dst, f.data(), n * 8, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
+    double* f = (double*)st;
+    host_parallel(n, [&](long long i0, long long i1) {
+      for (long long i = i0; i < i1; ++i) f[i] = host[i] - mean;
+    });
   }
+  HIPCHK(hipMemcpyAsync(dst, st, (size_t)n * c->rsize, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return XPFC_OK;
 }
 
@@ -449,10 +507,8 @@ extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
   if (rc) return rc;
   if (!host) return fail(XPFC_ERR_ARG, "null field");
   if ((rc = drain(c)) != XPFC_OK) return rc;  // pending records use the old mean
-  // Avg_calc.cu:3-18 mean, FP64, row-major order
-  double sum = 0.0;
-  for (long long i = 0; i < c->N; ++i) sum += host[i];
-  const double mean = sum / (double)c->N;
+  // Avg_calc.cu:3-18 mean, FP64 (fixed summation order)
+  const double mean = fixed_order_sum(host, c->N) / (double)c->N;
   if (!std::isfinite(mean)) return fail(XPFC_ERR_NONFINITE, "field contains NaN/Inf");
   c->inst[s].mean = mean;
   if ((rc = upload_inst(c, s))) return rc;
@@ -496,10 +552,14 @@ extern "C" int xpfc_inst_get_field(xpfc_ctx* c, int s, double* host) {
   const size_t n = (size_t)c->N;
   const void* src = inst_ptr(c, c->aux, s);
   if (c->prec == 1) {
-    std::vector<float> f(n);
-    HIPCHK(hipMemcpyAsync(f.data(), src, n * 4, hipMemcpyDeviceToHost, c->stream));
+    void* st = staging(c);
+    if (!st) return fail(XPFC_ERR_NOMEM, "pinned staging allocation failed");
+    HIPCHK(hipMemcpyAsync(st, src, n * 4, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));
-    for (size_t i = 0; i < n; ++i) host[i] = (double)f[i];
+    const float* f = (const float*)st;
+    host_parallel((long long)n, [&](long long i0, long long i1) {
+      for (long long i = i0; i < i1; ++i) host[i] = (double)f[i];
+    });
   } else {
     HIPCHK(hipMemcpyAsync(host, src, n * 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(hipStreamSynchronize(c->stream));

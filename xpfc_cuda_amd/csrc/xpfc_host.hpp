@@ -86,6 +86,7 @@ struct xpfc_ctx {
   int* d_flags = nullptr;    // [K] divergence probe results
   int npx_cap = 0, npz_cap = 0;
   hipStream_t stream = nullptr;
+  void* stage = nullptr;     // pinned host staging for field I/O (lazy)
   xpfc::KGeom g{};
   xpfc::Segs segX{}, segY{};
   int tc_x = 0, tc_y = 0, ntz = 256;  // pass variants
