@@ -166,11 +166,11 @@ def batched_run(steps=2000, ninst=64, seed=0):
     return res
 
 
-def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9):
+def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 << 25):
     """Poisson job stream through the continuous-batching service."""
     from xpfc_cuda_amd import serving as S
     jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d)
-    svc = S.Service(budget_bytes=budget)
+    svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells))
     try:
         svc.warm([(2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128), (3, 256)])
         rep = svc.run(jobs, realtime=True,
@@ -200,6 +200,8 @@ def main():
                     help="seconds of Poisson arrivals for the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
     ap.add_argument("--serve-3d", type=float, default=0.5, help="fraction of 3-D jobs")
+    ap.add_argument("--serve-class-cells", type=int, default=1 << 25,
+                    help="slot policy: cells per size-class launch")
     ap.add_argument("--only", choices=["batch", "serve"],
                     help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -226,8 +228,8 @@ def main():
         print(json.dumps({"batched": batched_run()}), flush=True)
         return
     if args.only == "serve":
-        print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d)}),
-              flush=True)
+        print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d,
+                                                  class_cells=args.serve_class_cells)}), flush=True)
         return
     n = args.n
     ms, wall, passes, samples, variants, io = gpu_run(n, 1, args.steps, args.warmup)
@@ -290,7 +292,8 @@ def main():
     if not args.no_batch:
         out["batched"] = batched_run()
     if args.serve > 0:
-        out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d)
+        out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d,
+                                     class_cells=args.serve_class_cells)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

@@ -83,10 +83,14 @@ def job_params(job: Job, precision: int = XPFC_FP32_MIXED) -> Params:
                   dcf_mode=XPFC_DCF_MAX, precision=precision, t_sample=0)
 
 
-def default_slots(dim: int, side: int) -> int:
-    """Slots of a size class: about 2^25 cells per class launch, 2..64 instances."""
-    cells = side ** dim
-    return int(max(2, min(64, (1 << 25) // cells)))
+def class_slots(cells_per_class: int = 1 << 25) -> Callable[[int, int], int]:
+    """Slot policy: about `cells_per_class` cells per class launch, 2..64 instances."""
+    def slots(dim: int, side: int) -> int:
+        return int(max(2, min(64, cells_per_class // side ** dim)))
+    return slots
+
+
+default_slots = class_slots()
 
 
 class SizeClass:
