@@ -69,8 +69,10 @@ def _run_both(op, lp, nsteps, amp=0.01, seed=0):
     return psi0, ref, rs, got, gs
 
 
+# (8, 128, 128) and (128, 128, 128) have planes of >= 64 KiB: padded plane stride
 @pytest.mark.parametrize("shape,nsteps", [((32, 32), 60), ((64, 64), 80), ((16, 16, 16), 40),
-                                          ((32, 16, 64), 30)])
+                                          ((32, 16, 64), 30), ((8, 128, 128), 20),
+                                          ((128, 128, 128), 10)])
 @pytest.mark.parametrize("prec", PRECS)
 def test_steps_vs_oracle_growth(shape, nsteps, prec):
     _flush("steps %s n=%d prec=%d" % (shape, nsteps, prec))

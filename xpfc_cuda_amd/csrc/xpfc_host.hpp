@@ -67,6 +67,8 @@ struct xpfc_ctx {
   int prec = 0;
   int n0 = 0, n1 = 0, n2 = 0, H = 0;
   long long N = 0, nmain = 0, nny = 0;
+  long long xs = 0;     // plane stride of the main block (complex elements, >= n1*H)
+  long long nyoff = 0;  // offset of the Nyquist plane = n0 * xs
   size_t csize = 0, rsize = 0;
   int K = 1;                 // instances
   long long stride = 0;      // complex elements per instance spectrum (nmain + nny)

@@ -23,9 +23,13 @@ struct Segs {
   int tiles0, tiles1;
 };
 
-// Wavenumber geometry of the packed spectrum.
+// Wavenumber geometry of the packed spectrum.  The main block is laid out
+// [n0][n1][H] with planes (fixed axis-0 index) xs complex elements apart,
+// xs >= n1*H (padding breaks the power-of-two plane stride); the Nyquist plane
+// [n0][n1] starts at nyoff = n0*xs.
 struct KGeom {
   int n0, n1, H;
+  long long xs, nyoff;
   double dk0, dk1, dk2;  // 2*pi/L per axis
   // Equal spacing on every axis (square / cubic boxes): k^2 = dksq * m with the
   // integer m = i^2 + j^2 + l^2, and C2 comes from a per-instance table over m.
@@ -58,6 +62,14 @@ struct ZArgs {
 };
 
 enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
+
+// Line placement of the real-space pass: line l = a*n1 + b starts at
+// a*xs + b*H complex elements (input and output may use different plane
+// strides, e.g. a caller's contiguous real field and a padded spectrum).
+struct ZLines {
+  int n1;
+  long long xs_in, xs_out;
+};
 
 // Per-instance constants of a context holding several simulations of one grid
 // (device array indexed by instance slot).
@@ -99,8 +111,8 @@ int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int t
                  const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
                  hipStream_t s);
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                 long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
-                 const void* twp, double* partial, bool sample, hipStream_t s);
+                 long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
+                 const void* tw, const void* twp, double* partial, bool sample, hipStream_t s);
 int zpass_blocks(int prec, int H, int ntz, long long nlines);
 int strided_blocks(int prec, int L, int tc, const Segs& segs);
 // out[slot * NSUMS + k]: the six sums of every listed instance
@@ -121,14 +133,18 @@ int launch_euler(int prec, const void* psi_k, const void* p2_k, const void* p3_k
 int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void* p2_k,
                     const void* p3_k, void* out, const KGeom& g, const Phys& ph, double a,
                     double b, int n2, hipStream_t s);
-int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s);
-int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s);
+// natural [n0][n1][H+1] <-> packed main/Nyquist layout (plane stride xs)
+int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, int n1, long long xs,
+                hipStream_t s);
+int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, int n1, long long xs,
+                  hipStream_t s);
 int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s);
 // noise initial condition: per-workgroup sums of the draws of cells 0..n-1
 // (nblocks returned, <= 4096), then out[c] = amp * (z_c - mean_z) as reals
 int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s);
-int launch_noise_fill(int prec, void* out, long long n, unsigned long long seed, double amp,
-                      double mean_z, hipStream_t s);
+// (the reals of cell c land at (c / plane) * pitch + c % plane)
+int launch_noise_fill(int prec, void* out, long long n, long long plane, long long pitch,
+                      unsigned long long seed, double amp, double mean_z, hipStream_t s);
 // flag[s] = 1 if instance s's spectrum (at s * stride) is finite at two probe modes
 int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
                  hipStream_t s);

@@ -87,26 +87,36 @@ __global__ void k_residual(const cpx<T>* __restrict__ kp1, const cpx<T>* __restr
   }
 }
 
+// packed main-block offset of line l = a*n1 + b (plane stride xs)
+__device__ __forceinline__ long long line_off(long long l, int n1, long long xs, int H) {
+  const long long a = l / n1;
+  return a * xs + (l - a * n1) * H;
+}
+
 template <typename T>
-__global__ void k_pack(const cpx<T>* __restrict__ nat, cpx<T>* __restrict__ mainny, int n0n1, int H) {
+__global__ void k_pack(const cpx<T>* __restrict__ nat, cpx<T>* __restrict__ mainny, int n0n1, int H,
+                       int n1, long long xs) {
   const long long tot = (long long)n0n1 * (H + 1);
+  const long long nyoff = (long long)(n0n1 / n1) * xs;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
     const int kz = (int)(e % (H + 1));
     const long long l = e / (H + 1);
-    if (kz < H) mainny[l * H + kz] = nat[e];
-    else mainny[(long long)n0n1 * H + l] = nat[e];
+    if (kz < H) mainny[line_off(l, n1, xs, H) + kz] = nat[e];
+    else mainny[nyoff + l] = nat[e];
   }
 }
 
 template <typename T>
-__global__ void k_unpack(const cpx<T>* __restrict__ mainny, cpx<T>* __restrict__ nat, int n0n1, int H) {
+__global__ void k_unpack(const cpx<T>* __restrict__ mainny, cpx<T>* __restrict__ nat, int n0n1, int H,
+                         int n1, long long xs) {
   const long long tot = (long long)n0n1 * (H + 1);
+  const long long nyoff = (long long)(n0n1 / n1) * xs;
   for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < tot;
        e += (long long)gridDim.x * blockDim.x) {
     const int kz = (int)(e % (H + 1));
     const long long l = e / (H + 1);
-    nat[e] = kz < H ? mainny[l * H + kz] : mainny[(long long)n0n1 * H + l];
+    nat[e] = kz < H ? mainny[line_off(l, n1, xs, H) + kz] : mainny[nyoff + l];
   }
 }
 
@@ -162,12 +172,16 @@ __global__ __launch_bounds__(256) void k_noise_sum(long long n, uint32_t k0, uin
   block_sum<256, 1>(a, partial + blockIdx.x);
 }
 
-// out[c] = amp * (z_c - mean_z): the fluctuation about the mean psi0
+// amp * (z_c - mean_z), the fluctuation about the mean psi0, as reals at
+// (c / plane) * pitch + c % plane
 template <typename T>
-__global__ __launch_bounds__(256) void k_noise_fill(T* __restrict__ out, long long n, uint32_t k0,
-                                                    uint32_t k1, double amp, double mean_z) {
-  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256)
-    out[c] = (T)(amp * (randn_cell((unsigned long long)c, k0, k1) - mean_z));
+__global__ __launch_bounds__(256) void k_noise_fill(T* __restrict__ out, long long n, long long plane,
+                                                    long long pitch, uint32_t k0, uint32_t k1,
+                                                    double amp, double mean_z) {
+  for (long long c = (long long)blockIdx.x * 256 + threadIdx.x; c < n; c += (long long)gridDim.x * 256) {
+    const long long a = c / plane;
+    out[a * pitch + (c - a * plane)] = (T)(amp * (randn_cell((unsigned long long)c, k0, k1) - mean_z));
+  }
 }
 
 int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s) {
@@ -178,15 +192,15 @@ int launch_noise_sum(long long n, unsigned long long seed, double* partial, int*
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_noise_fill(int prec, void* out, long long n, unsigned long long seed, double amp,
-                      double mean_z, hipStream_t s) {
+int launch_noise_fill(int prec, void* out, long long n, long long plane, long long pitch,
+                      unsigned long long seed, double amp, double mean_z, hipStream_t s) {
   const int nb = (int)std::min<long long>(8192, std::max<long long>(1, (n + 255) / 256));
   if (prec == 1)
-    hipLaunchKernelGGL(k_noise_fill<float>, dim3(nb), dim3(256), 0, s, (float*)out, n, (uint32_t)seed,
-                       (uint32_t)(seed >> 32), amp, mean_z);
+    hipLaunchKernelGGL(k_noise_fill<float>, dim3(nb), dim3(256), 0, s, (float*)out, n, plane, pitch,
+                       (uint32_t)seed, (uint32_t)(seed >> 32), amp, mean_z);
   else
-    hipLaunchKernelGGL(k_noise_fill<double>, dim3(nb), dim3(256), 0, s, (double*)out, n, (uint32_t)seed,
-                       (uint32_t)(seed >> 32), amp, mean_z);
+    hipLaunchKernelGGL(k_noise_fill<double>, dim3(nb), dim3(256), 0, s, (double*)out, n, plane, pitch,
+                       (uint32_t)seed, (uint32_t)(seed >> 32), amp, mean_z);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
@@ -275,25 +289,27 @@ int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, hipStream_t s) {
+int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, int n1, long long xs,
+                hipStream_t s) {
   const long long n = (long long)n0n1 * (H + 1);
   if (prec == 1)
     hipLaunchKernelGGL(k_pack<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const cpx<float>*)natural, (cpx<float>*)main_ny, n0n1, H);
+                       (const cpx<float>*)natural, (cpx<float>*)main_ny, n0n1, H, n1, xs);
   else
     hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const cpx<double>*)natural, (cpx<double>*)main_ny, n0n1, H);
+                       (const cpx<double>*)natural, (cpx<double>*)main_ny, n0n1, H, n1, xs);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 
-int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, hipStream_t s) {
+int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, int n1, long long xs,
+                  hipStream_t s) {
   const long long n = (long long)n0n1 * (H + 1);
   if (prec == 1)
     hipLaunchKernelGGL(k_unpack<float>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const cpx<float>*)main_ny, (cpx<float>*)natural, n0n1, H);
+                       (const cpx<float>*)main_ny, (cpx<float>*)natural, n0n1, H, n1, xs);
   else
     hipLaunchKernelGGL(k_unpack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
-                       (const cpx<double>*)main_ny, (cpx<double>*)natural, n0n1, H);
+                       (const cpx<double>*)main_ny, (cpx<double>*)natural, n0n1, H, n1, xs);
   return hipGetLastError() == hipSuccess ? 0 : -2;
 }
 

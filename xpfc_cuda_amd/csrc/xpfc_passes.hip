@@ -227,7 +227,7 @@ template <int H, int E> struct ZIdx {
 template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE>
 __global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
 void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
-             cpx<T>* __restrict__ out_ny, long long nlines, Batch bt,
+             cpx<T>* __restrict__ out_ny, long long nlines, ZLines zl, Batch bt,
              const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
              double* __restrict__ partial) {
   using C = cpx<T>;
@@ -253,12 +253,14 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   const int g = tid / TPL, t = tid - g * TPL;
   const long long line = (long long)blockIdx.x * G + g;
   const bool act = line < nlines;
+  const long long la = line / zl.n1, lb = line - la * zl.n1;
+  const long long off_in = la * zl.xs_in + lb * H, off_out = la * zl.xs_out + lb * H;
   const ZIdx<H, E> idx{g};
   double acc[ZPART] = {0.0, 0.0, 0.0, 0.0};
   C v[E];
 
   if constexpr (MODE == Z_STEP || MODE == Z_C2R) {
-    const C* sp = reinterpret_cast<const C*>(in) + line * H;
+    const C* sp = reinterpret_cast<const C*>(in) + off_in;
     const T s = (T)za.scale;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -273,7 +275,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
     }
     fft_line<T, H, E, +1>(v, t, lds, idx, twl);
     if constexpr (MODE == Z_C2R) {
-      C* rp = reinterpret_cast<C*>(out) + line * H;  // real pairs
+      C* rp = reinterpret_cast<C*>(out) + off_out;  // real pairs
 #pragma unroll
       for (int m = 0; m < E; ++m) {
         const int k = t + m * TPL;
@@ -309,7 +311,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
     }
   } else {
     // input: the fluctuation psi - mean as real pairs (x[2k], x[2k+1])
-    const C* rp = reinterpret_cast<const C*>(in) + line * H;
+    const C* rp = reinterpret_cast<const C*>(in) + off_in;
     const T a = (T)za.a, b = (T)za.b;
 #pragma unroll
     for (int m = 0; m < E; ++m) {
@@ -329,7 +331,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
 #pragma unroll
   for (int m = 0; m < E; ++m) lds[idx(t + m * TPL)] = v[m];
   __syncthreads();
-  C* op = reinterpret_cast<C*>(out) + line * H;
+  C* op = reinterpret_cast<C*>(out) + off_out;
   const T half = (T)0.5;
 #pragma unroll
   for (int m = 0; m < E; ++m) {
@@ -411,7 +413,8 @@ __global__ __launch_bounds__(256) void k_kspace_sums(const cpx<T>* __restrict__ 
       const double k2 = kx * kx + ky * ky + kzv * kzv;
       c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
     }
-    const cpx<T> v = psi[e];
+    const cpx<T> v = psi[e < nmain ? (long long)ka * g.xs + (long long)kb * g.H + kz
+                                  : g.nyoff + (e - nmain)];
     acc[0] += w * c2 * ((double)v.x * v.x + (double)v.y * v.y);
   }
   block_sum<256, XPART>(acc, partial + (long long)blockIdx.x * XPART);
@@ -562,25 +565,25 @@ int zpass_blocks(int prec, int H, int ntz, long long nlines) {
 
 template <typename T, int H, int NZ, bool FRE, int MODE>
 static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, long long nlines,
-                    const Batch& bt, int ny, const void* tw, const void* twp, double* partial,
-                    bool sample, hipStream_t s) {
+                    const ZLines& zl, const Batch& bt, int ny, const void* tw, const void* twp,
+                    double* partial, bool sample, hipStream_t s) {
   constexpr int G = NZ / z_tpl<T, H>();
   const dim3 grid((unsigned)((nlines + G - 1) / G), ny);
   if (sample)
     hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, true>), grid, dim3(NZ), 0, s, in,
-                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, bt,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, zl, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
   else
     hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, false>), grid, dim3(NZ), 0, s, in,
-                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, bt,
+                       (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, zl, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
 }
 
 // ntz: workgroup size (64..512); +ZFREE_REGS drops the occupancy hint
 template <typename T>
 static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                   long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
-                   const void* twp, double* partial, bool sample, hipStream_t s) {
+                   long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
+                   const void* tw, const void* twp, double* partial, bool sample, hipStream_t s) {
   if (ny == 0) return 0;
   const bool fre = (ntz & ZFREE_REGS) != 0;
   ntz &= ~ZFREE_REGS;
@@ -588,9 +591,9 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
     XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
       if constexpr (z_ok<T, LL, NZ>()) {
         if (fre)
-          zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s);
+          zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
         else
-          zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s);
+          zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
       } else {
         return -3;
       }
@@ -600,9 +603,9 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
     XPFC_LSWITCH(H, {
       constexpr int NZ = 256;
       switch (mode) {
-        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, false, s); break;
-        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, false, s); break;
-        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, bt, ny, tw, twp, partial, sample, s); break;
+        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
+        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
+        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s); break;
         default: return -1;
       }
     });
@@ -611,11 +614,11 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
 }
 
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
-                 long long nlines, int H, int ntz, const Batch& bt, int ny, const void* tw,
-                 const void* twp, double* partial, bool sample, hipStream_t s) {
+                 long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
+                 const void* tw, const void* twp, double* partial, bool sample, hipStream_t s) {
   return prec == 1
-             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, ntz, bt, ny, tw, twp, partial, sample, s)
-             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, ntz, bt, ny, tw, twp, partial, sample, s);
+             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, s)
+             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, s);
 }
 
 int launch_finalize(const double* px, int nx, const double* pz, int nz, const Batch& bt, int ny,
