@@ -67,7 +67,7 @@ enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
 // a*xs + b*H complex elements (input and output may use different plane
 // strides, e.g. a caller's contiguous real field and a padded spectrum).
 struct ZLines {
-  int n1;
+  int n1;  // a power of two
   long long xs_in, xs_out;
 };
 

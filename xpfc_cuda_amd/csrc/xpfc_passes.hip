@@ -253,7 +253,8 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   const int g = tid / TPL, t = tid - g * TPL;
   const long long line = (long long)blockIdx.x * G + g;
   const bool act = line < nlines;
-  const long long la = line / zl.n1, lb = line - la * zl.n1;
+  const int lg1 = __builtin_ctz(zl.n1);  // n1 is a power of two
+  const long long la = line >> lg1, lb = line & (zl.n1 - 1);
   const long long off_in = la * zl.xs_in + lb * H, off_out = la * zl.xs_out + lb * H;
   const ZIdx<H, E> idx{g};
   double acc[ZPART] = {0.0, 0.0, 0.0, 0.0};
