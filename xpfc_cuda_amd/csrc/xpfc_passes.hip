@@ -120,9 +120,13 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
-  C v[E];
+  // both input streams are read up front (their latency hides behind the
+  // forward transform), the two output streams written in bursts after it
+  C v[E], ps[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+#pragma unroll
+  for (int m = 0; m < E; ++m) ps[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
   stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
 
@@ -143,13 +147,11 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   // in the working precision.  Each element's FP64 chain is fenced off with a
   // scheduling barrier so the unrolled iterations do not interleave and the
   // register footprint stays that of one element.
-  C nxt = act ? pp[t * rs] : mk<C>(0, 0);
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     __builtin_amdgcn_sched_barrier(0);
     const int ka = t + m * (L / E);
-    const C old = nxt;
-    if (m + 1 < E) nxt = act ? pp[(ka + L / E) * rs] : mk<C>(0, 0);  // prefetch
+    const C old = ps[m];
     const int ia = signed_index(ka, g.n0);
     double k2, c2;
     if constexpr (LUT) {
@@ -181,10 +183,13 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
                         + ph.Mres * k2 * ((double)old.y + (double)nh.y - c2 * (double)old.y);
       acc[1] += wgt * (rx * rx + ry * ry);
     }
-    if (act) pp[ka * rs] = nv;
     v[m] = nv;
   }
   __builtin_amdgcn_sched_barrier(0);
+  if (act) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) pp[(t + m * (L / E)) * rs] = v[m];
+  }
 
   fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
   if (act) {
