@@ -120,6 +120,12 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
+  // wavenumber of this column
+  int kb, kz;
+  if (loc.sg->kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
+  else { kb = loc.col; kz = g.H; }
+  const int jb = signed_index(kb, g.n1);
+  const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
   // both input streams are read up front (their latency hides behind the
   // forward transform), the two output streams written in bursts after it
   C v[E], ps[E];
@@ -127,15 +133,17 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
 #pragma unroll
   for (int m = 0; m < E; ++m) ps[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  double c2t[LUT ? E : 1];
+  if constexpr (LUT) {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ia = signed_index(t + m * (L / E), g.n0);
+      c2t[m] = c2_lookup(ia * ia + myz, ph);
+    }
+  }
   stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
 
-  // wavenumber of this column
-  int kb, kz;
-  if (loc.sg->kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
-  else { kb = loc.col; kz = g.H; }
-  const int jb = signed_index(kb, g.n1);
-  const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
   const double ky = g.dk1 * (double)jb;
   const double kzv = g.dk2 * (double)kz;
   const double kyz2 = ky * ky + kzv * kzv;
@@ -155,9 +163,8 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
     const int ia = signed_index(ka, g.n0);
     double k2, c2;
     if constexpr (LUT) {
-      const int mm = ia * ia + myz;
-      k2 = g.dksq * (double)mm;
-      c2 = c2_lookup(mm, ph);
+      k2 = g.dksq * (double)(ia * ia + myz);
+      c2 = c2t[m];
     } else {
       const double kx = g.dk0 * (double)ia;
       k2 = kx * kx + kyz2;
