@@ -275,11 +275,24 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   if constexpr (MODE == Z_STEP || MODE == Z_C2R) {
     const C* sp = reinterpret_cast<const C*>(in) + off_in;
     const T s = (T)za.scale;
+    // The pre-twiddle pairs X[k] with X[H-k].  For t >= 1, X[H-k] (k = t + m*TPL)
+    // is element E-1-m of lane TPL-t of the same line; for t = 0 it is this
+    // lane's element E-m (and the Nyquist bin for m = 0).  The line is read
+    // once and the mirror exchanged across lanes.
+    C xo[E];
+#pragma unroll
+    for (int m = 0; m < E; ++m) xo[m] = act ? sp[t + m * TPL] : mk<C>(0, 0);
+    const C xnyq = (act && t == 0) ? in_ny[line] : mk<C>(0, 0);
+    const int lane = threadIdx.x & 63;
+    const int src = lane - t + ((TPL - t) & (TPL - 1));
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int k = t + m * TPL;
-      C xk = act ? sp[k] : mk<C>(0, 0);
-      C xm = act ? (k == 0 ? in_ny[line] : sp[H - k]) : mk<C>(0, 0);
+      C xk = xo[m];
+      C xm;
+      xm.x = __shfl(xo[E - 1 - m].x, src, 64);
+      xm.y = __shfl(xo[E - 1 - m].y, src, 64);
+      if (t == 0) xm = (m == 0) ? xnyq : xo[(E - m) & (E - 1)];
       if (k == 0) { xk.y = 0; xm.y = 0; }  // DC and Nyquist bins are real (C2R)
       const C e = mk<C>(xk.x + xm.x, xk.y - xm.y);
       const C d = mk<C>(xk.x - xm.x, xk.y + xm.y);
