@@ -103,6 +103,38 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
                       % (done, n, workers, platform.processor() or platform.machine())}
 
 
+def config_runs():
+    """The other single-simulation configurations: C1 (2-D 256^2 square lattice,
+    FP64, 1000 steps, FE/RMSE every 100) and C2 (3-D 256^3 bcc, FP64, 1000
+    steps), timed with hipEvents after 20 warm-up steps."""
+    import xpfc_cuda_amd as X
+    k1 = 2.0 * math.pi
+    out = {}
+    cases = {
+        "c1_2d_256_fp64_square": dict(n=(256, 256), L=(32.0, 32.0), sigma=0.1, psi0=0.05, seed=0,
+                                      peaks=((k1, 1.0, 1.0), (k1 * math.sqrt(2.0), 1.0, 1.0))),
+        "c2_3d_256_fp64_bcc": dict(n=(256, 256, 256), L=(32.0, 32.0, 32.0), sigma=0.12, psi0=0.04,
+                                   seed=1, peaks=((k1, 1.0, 1.0),)),
+    }
+    for name, cfg in cases.items():
+        p = X.Params(n=cfg["n"], L=cfg["L"], dt=0.5, Mo=1.0, eps=0.0, a=1.0, b=1.0 / 3.0,
+                     sigma=cfg["sigma"], psi0=cfg["psi0"], peaks=cfg["peaks"],
+                     dcf_mode=X.XPFC_DCF_MAX, precision=X.XPFC_FP64, t_sample=100)
+        with X.Context(p) as ctx:
+            ctx.initialize_noise(0.01, cfg["seed"])
+            ctx.step(20)
+            ctx.sync()
+            ms = ctx.time_steps(1000)
+            cells = 1
+            for m in cfg["n"]:
+                cells *= m
+            out[name] = {"steps_per_s": round(1000.0 / (ms * 1e-3), 1),
+                         "ms_per_step": round(ms / 1000.0, 5),
+                         "cell_updates_per_s": cells * 1000.0 / (ms * 1e-3),
+                         "last_sample": ctx.samples()[-1].line().strip()}
+    return out
+
+
 def batched_run(steps=2000, ninst=64, seed=0):
     """The 64-instance batch: one multi-instance context per grid side, all
     classes queued at once on their own streams; wall time of the whole job."""
@@ -202,7 +234,7 @@ def main():
     ap.add_argument("--serve-3d", type=float, default=0.5, help="fraction of 3-D jobs")
     ap.add_argument("--serve-class-cells", type=int, default=1 << 25,
                     help="slot policy: cells per size-class launch")
-    ap.add_argument("--only", choices=["batch", "serve"],
+    ap.add_argument("--only", choices=["batch", "serve", "configs"],
                     help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run --steps unsampled steps and one field read-back (its D2D copy of "
@@ -224,6 +256,9 @@ def main():
               "(the workload would shard as x-slabs with two all-to-all transposes per step)",
               file=sys.stderr)
         sys.exit(2)
+    if args.only == "configs":
+        print(json.dumps({"configs": config_runs()}), flush=True)
+        return
     if args.only == "batch":
         print(json.dumps({"batched": batched_run()}), flush=True)
         return
@@ -289,6 +324,7 @@ def main():
             tr = json.load(f)["k_xstep"].get(str(variants[0]))
         if tr:
             out["roofline"]["traffic"] = tr["traffic_bytes"]
+    out["configs"] = config_runs()
     if not args.no_batch:
         out["batched"] = batched_run()
     if args.serve > 0:
