@@ -185,3 +185,25 @@ def test_read_input_restart_box_from_vtk(tmp_path, monkeypatch):
     monkeypatch.chdir(tmp_path)
     p = X.read_input(str(tmp_path / "pfc.input"))
     assert p.iconfig == 10 and p.L[:2] == (40.0, 24.0)
+
+
+@pytest.mark.parametrize("n,L,msg", [
+    ((48, 32), (6.0, 4.0), "power of two"),      # not a power of two
+    ((4, 32), (0.5, 4.0), "power of two"),       # below the smallest line
+    ((32, 32), (0.0, 4.0), "box length"),        # empty box
+])
+def test_create_rejects_bad_grids_before_touching_a_device(n, L, msg):
+    p = X.Params(n=n, L=L)
+    with pytest.raises(X.XPFCError) as e:
+        X.Context(p)
+    assert e.value.status == X.api.XPFC_ERR_ARG and msg in str(e.value)
+
+
+def test_instance_bytes_counts_three_spectra():
+    p32 = X.Params(n=(64, 64, 64), L=(8.0, 8.0, 8.0), precision=X.XPFC_FP32_MIXED)
+    p64 = X.Params(n=(64, 64, 64), L=(8.0, 8.0, 8.0), precision=X.XPFC_FP64)
+    spectrum32 = 64 * 64 * 33 * 8
+    b32, b64 = X.instance_bytes(p32), X.instance_bytes(p64)
+    assert 3 * spectrum32 <= b32 < 3 * spectrum32 + (1 << 20)
+    assert 3 * 2 * spectrum32 <= b64 < 3 * 2 * spectrum32 + (1 << 20)
+    assert X.instance_bytes(X.Params(n=(48, 48), L=(6.0, 6.0))) == 0
