@@ -34,6 +34,11 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // Plane stride (complex elements) of the packed main block: planes of 64 KiB
 // or more get 128 bytes of padding, so the axis-0 pass does not stride by a
 // large power of two (measured: the 512^3 FP32 axis-0 pass 10 % faster).
+// Fields of this many bytes per spectrum buffer (all instances) and more do
+// not fit the 256 MiB last-level cache: their axis-1 passes use streaming
+// hints.
+constexpr size_t STREAM_BYTES = (size_t)256 << 20;
+
 long long plane_stride(int n1, int H, size_t csize) {
   const long long plane = (long long)n1 * H;
   return plane * (long long)csize >= 65536 ? plane + (long long)(128 / csize) : plane;
@@ -317,6 +322,7 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, c->segY) : 0;
   if (c->tc_x <= 0 || (c->n1 > 1 && c->tc_y <= 0)) return bail(fail(XPFC_ERR_ARG, "unsupported grid"));
   c->ntz = 256;
+  c->nt_y = sbytes >= STREAM_BYTES;
   // partial buffers sized for the largest variant (narrowest tile, smallest block)
   c->npx_cap = 4096;
   for (int tc : {4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, c->segX));
@@ -466,14 +472,16 @@ extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
 namespace {
 
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->stream));
-  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, bt, ny, c->stream));
+  if (c->n1 > 1)
+    LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->nt_y, c->stream));
+  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, bt, ny, false, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
-  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, bt, ny, c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->stream));
+  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
+  if (c->n1 > 1)
+    LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->nt_y, c->stream));
   return XPFC_OK;
 }
 
@@ -721,12 +729,14 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
                                c->px, sample, c->stream);
   LCHK(nbx);
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->stream));
+  if (c->n1 > 1)
+    LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->nt_y, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
   LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl, c->H,
                     zl_own(c), c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
-  if (c->n1 > 1) LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->stream));
+  if (c->n1 > 1)
+    LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->nt_y, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   if (sample)
     LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->prec, c->H, c->ntz, nl), bt, ny,
@@ -895,7 +905,7 @@ int fe_now(xpfc_ctx* c, int s, double* F, double* fm, double* mean_rs) {
 // allocation, so the choice never changes the computed field.
 // ---------------------------------------------------------------------------
 std::mutex g_tune_mu;
-std::map<std::array<int, 5>, std::array<int, 3>> g_tune_cache;
+std::map<std::array<int, 6>, std::array<int, 3>> g_tune_cache;
 
 int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps, float* ms) {
   const long long nl = (long long)c->n0 * c->n1;
@@ -903,7 +913,7 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
   auto launch = [&]() -> int {
     if (which == 0)
       return launch_xstep(c->prec, a, b, c->segX, c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
-    if (which == 1) return launch_strided(c->prec, a, c->segY, c->n1, v, -1, c->tw1, bt, ny, c->stream);
+    if (which == 1) return launch_strided(c->prec, a, c->segY, c->n1, v, -1, c->tw1, bt, ny, c->nt_y, c->stream);
     return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, zl_own(c), v, bt, ny,
                         c->twz, c->twp, c->pz, false, c->stream);
   };
@@ -943,7 +953,7 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
 int autotune(xpfc_ctx* c) {
   // tune with up to 8 instances side by side (the batched launch shape)
   const int ny = std::min(c->K, 8);
-  const std::array<int, 5> key{c->prec, c->n0, c->n1, c->n2, ny};
+  const std::array<int, 6> key{c->prec, c->n0, c->n1, c->n2, ny, c->nt_y ? 1 : 0};
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     auto it = g_tune_cache.find(key);

@@ -25,6 +25,26 @@ __device__ __forceinline__ double c2_lookup(int m, const Phys& ph) {
   return m < ph.mcut ? ph.c2tab[m] : 0.0;
 }
 
+// Complex loads and stores with the non-temporal (streaming) cache hint: the
+// line is not kept in the caches for reuse.
+template <typename C> __device__ __forceinline__ C load_nt(const C* p) {
+  using S = decltype(p->x);
+  typedef S V __attribute__((ext_vector_type(2)));
+  const V v = __builtin_nontemporal_load(reinterpret_cast<const V*>(p));
+  C r;
+  r.x = v.x;
+  r.y = v.y;
+  return r;
+}
+template <typename C> __device__ __forceinline__ void store_nt(C* p, C c) {
+  using S = decltype(p->x);
+  typedef S V __attribute__((ext_vector_type(2)));
+  V v;
+  v.x = c.x;
+  v.y = c.y;
+  __builtin_nontemporal_store(v, reinterpret_cast<V*>(p));
+}
+
 // FFT_freq.cu:17-24 index -> signed wavenumber index (Nyquist positive)
 __device__ __forceinline__ int signed_index(int i, int n) { return i <= n / 2 ? i : i - n; }
 

@@ -104,9 +104,10 @@ constexpr int NSUMS = 6;
 // workgroup size (64..512) of the real-space pass; a combination a size does
 // not support returns -3.  launch_xstep returns its workgroup count (= number
 // of X partials) on success.
-// `ny` is the number of instances covered (entries of bt.slots used).
+// `ny` is the number of instances covered (entries of bt.slots used).  `nt`
+// selects the streaming-hint loads and stores of the strided pass.
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
-                   const Batch& bt, int ny, hipStream_t s);
+                   const Batch& bt, int ny, bool nt, hipStream_t s);
 int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
                  const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
                  hipStream_t s);

@@ -69,7 +69,13 @@ __device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, in
 // ---------------------------------------------------------------------------
 // strided pass: forward (DIR=-1) or inverse (DIR=+1) transform of every line
 // ---------------------------------------------------------------------------
-template <typename T, int L, int TC, int DIR>
+// NT: loads and stores carry the streaming hint.  For fields larger than the
+// last-level cache the passes that follow the axis-1 passes then run faster
+// (measured at 512^3: the axis-0 pass after the forward axis-1 pass -6 %, the
+// real-space pass after the inverse one -6 %, the forward axis-1 pass itself
+// +7 %; the step -2.3 % in FP32, -4 % in FP64); for fields that fit the cache
+// the hint costs up to 15 %.
+template <typename T, int L, int TC, int DIR, bool NT>
 __global__ __launch_bounds__((tile_threads<L, TC>()))
 void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
   using C = cpx<T>;
@@ -83,12 +89,19 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   C* p = buf + loc.off;
   C v[E];
 #pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = loc.act ? p[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  for (int m = 0; m < E; ++m) {
+    C* q = p + (t + m * (L / E)) * rs;
+    v[m] = loc.act ? (NT ? load_nt(q) : *q) : mk<C>(0, 0);
+  }
   stage_table<tile_threads<L, TC>()>(twl, tw, L);  // read after the first exchange's barrier
   fft_line<T, L, E, DIR>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
   if (loc.act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) p[(t + m * (L / E)) * rs] = v[m];
+    for (int m = 0; m < E; ++m) {
+      C* q = p + (t + m * (L / E)) * rs;
+      if constexpr (NT) store_nt(q, v[m]);
+      else *q = v[m];
+    }
   }
 }
 
@@ -490,20 +503,27 @@ static Segs fill_tiles(const Segs& in, int TC) {
   return s;
 }
 
+template <typename T, int LL, int TT, int DIR>
+static void strided_go(void* buf, const Segs& sg, int nb, const void* tw, const Batch& bt, int ny,
+                       bool nt, hipStream_t s) {
+  if (nt)
+    hipLaunchKernelGGL((k_strided<T, LL, TT, DIR, true>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
+                       s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
+  else
+    hipLaunchKernelGGL((k_strided<T, LL, TT, DIR, false>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
+                       s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
+}
+
 template <typename T>
 static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const void* tw,
-                     const Batch& bt, int ny, hipStream_t s) {
+                     const Batch& bt, int ny, bool nt, hipStream_t s) {
   XPFC_LSWITCH(L, XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       const int nb = sg.tiles0 + sg.tiles1;
       if (nb == 0 || ny == 0) return 0;
-      if (dir < 0)
-        hipLaunchKernelGGL((k_strided<T, LL, TT, -1>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
-                           s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
-      else
-        hipLaunchKernelGGL((k_strided<T, LL, TT, +1>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0,
-                           s, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
+      if (dir < 0) strided_go<T, LL, TT, -1>(buf, sg, nb, tw, bt, ny, nt, s);
+      else strided_go<T, LL, TT, +1>(buf, sg, nb, tw, bt, ny, nt, s);
     } else {
       return -3;
     }
@@ -512,9 +532,9 @@ static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const
 }
 
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
-                   const Batch& bt, int ny, hipStream_t s) {
-  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, bt, ny, s)
-                   : strided_T<double>(buf, segs, L, tc, dir, tw, bt, ny, s);
+                   const Batch& bt, int ny, bool nt, hipStream_t s) {
+  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, bt, ny, nt, s)
+                   : strided_T<double>(buf, segs, L, tc, dir, tw, bt, ny, nt, s);
 }
 
 template <typename T>
