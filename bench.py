@@ -309,8 +309,10 @@ def main():
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
         out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
-    out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1],
-                       "real_space_block": variants[2]}
+    out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1] & ~1,
+                       "axis1_xcd_pairs": bool(variants[1] & 1),
+                       "real_space_block": variants[2] & ~1,
+                       "real_space_free_regs": bool(variants[2] & 1)}
     if io:
         # a 1000-step job that takes its field from host memory and returns it
         io_ms = io["get_field_ms"] + io["set_field_ms"]

@@ -144,10 +144,10 @@ int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
  * average ms per whole step. */
 int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
 /* Pass variants: v[0] = tile width (columns per workgroup, 4/8/16) of the
- * axis-0 pass (plus 1: register budget for two workgroups per CU), v[1] =
- * of the axis-1 passes, v[2] = workgroup size (64..512) of the real-space pass
- * (plus 1: no occupancy hint).  Variants regroup lines into workgroups or
- * change register allocation, and never
+ * axis-0 pass, v[1] = that of the axis-1 passes (plus 1: workgroups dealt to
+ * the XCDs in runs of two tiles), v[2] = workgroup size (64..512) of the
+ * real-space pass (plus 1: no occupancy hint).  Variants regroup lines into
+ * workgroups, reorder workgroups or change register allocation, and never
  * change the computed field.  xpfc_autotune times every variant on scratch
  * buffers, keeps the fastest (cached per precision and grid for the process)
  * and returns the choice in v (may be NULL); xpfc_step runs it on first use

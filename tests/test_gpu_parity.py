@@ -171,14 +171,17 @@ def test_mass_conserved_fp32():
         assert math.isfinite(F)
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (32, 64, 32)])
+@pytest.mark.parametrize("shape", [(64, 64), (32, 64, 32), (16, 128, 128)])
 @pytest.mark.parametrize("prec", PRECS)
 def test_variants_give_bitwise_identical_fields(shape, prec):
-    """Tile width / block size only regroup lines: the field must not change."""
+    """Tile width, workgroup order and block size only regroup lines or
+    reorder workgroups: the field must not change.  (16, 128, 128) has enough
+    axis-0 tiles for the run-dealt XCD order."""
     op, lp = growth_case(shape, prec=prec, t_sample=0)
     psi0 = O.noise_ic(op, 0.01, 9)
     outs = []
-    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (16, 0, 257), (8, 0, 65)]:
+    for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (16, 0, 257), (8, 0, 65),
+              (4, 5, 64), (16, 17, 257)]:
         with Context(lp) as ctx:
             try:
                 ctx.set_variants(*v)

@@ -39,6 +39,13 @@ bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 // hints.
 constexpr size_t STREAM_BYTES = (size_t)256 << 20;
 
+// Workgroup order over the XCDs (log2 of the runs of tiles dealt to each XCD
+// in turn).  Measured against one contiguous run per XCD: the axis-0 pass with
+// runs of 64 tiles -4 % at 512^3 FP32 and FP64, -6 % at 256^3 FP64; the
+// axis-1 passes with runs of two -6 % at 512^3 FP32 but +3 % at 512^3 FP64 and
+// +5 % at 256^3 FP64, so that order is a tuned variant (YRUNS_PAIRED).
+constexpr int XCD_RUN_X = 6, XCD_RUN_Y = 1;
+
 long long plane_stride(int n1, int H, size_t csize) {
   const long long plane = (long long)n1 * H;
   return plane * (long long)csize >= 65536 ? plane + (long long)(128 / csize) : plane;
@@ -318,6 +325,8 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   c->segX.s[1] = Seg{c->nyoff, 0, c->n1, 1, c->n1, 1, 0, 1};
   c->segY.s[0] = Seg{0, c->xs, c->H, 1, c->H, c->n0, 0, 0};
   c->segY.s[1] = Seg{c->nyoff, 0, 1, c->n1, c->n0, 1, 0, 1};
+  c->segX.lgc = XCD_RUN_X;
+  c->segY.lgc = -1;  // set per launch from the axis-1 variant
   c->tc_x = default_tc(c->prec, c->n0, c->segX);
   c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, c->segY) : 0;
   if (c->tc_x <= 0 || (c->n1 > 1 && c->tc_y <= 0)) return bail(fail(XPFC_ERR_ARG, "unsupported grid"));
@@ -471,17 +480,23 @@ extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// axis-1 pass with variant v: tile width and workgroup order
+int launch_y(xpfc_ctx* c, void* spec, int v, int dir, const Batch& bt, int ny) {
+  Segs sg = c->segY;
+  sg.lgc = (v & YRUNS_PAIRED) ? XCD_RUN_Y : -1;
+  return launch_strided(c->prec, spec, sg, c->n1, v & ~YRUNS_PAIRED, dir, c->tw1, bt, ny, c->nt_y,
+                        c->stream);
+}
+
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
-  if (c->n1 > 1)
-    LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->nt_y, c->stream));
+  if (c->n1 > 1) LCHK(launch_y(c, spec, c->tc_y, -1, bt, ny));
   if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, bt, ny, false, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
   LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
-  if (c->n1 > 1)
-    LCHK(launch_strided(c->prec, spec, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->nt_y, c->stream));
+  if (c->n1 > 1) LCHK(launch_y(c, spec, c->tc_y, +1, bt, ny));
   return XPFC_OK;
 }
 
@@ -729,14 +744,12 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
                                c->px, sample, c->stream);
   LCHK(nbx);
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-  if (c->n1 > 1)
-    LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, +1, c->tw1, bt, ny, c->nt_y, c->stream));
+  if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, +1, bt, ny));
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
   LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl, c->H,
                     zl_own(c), c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
-  if (c->n1 > 1)
-    LCHK(launch_strided(c->prec, c->work, c->segY, c->n1, c->tc_y, -1, c->tw1, bt, ny, c->nt_y, c->stream));
+  if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, -1, bt, ny));
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
   if (sample)
     LCHK(launch_finalize(c->px, nbx, c->pz, zpass_blocks(c->prec, c->H, c->ntz, nl), bt, ny,
@@ -913,7 +926,7 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
   auto launch = [&]() -> int {
     if (which == 0)
       return launch_xstep(c->prec, a, b, c->segX, c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
-    if (which == 1) return launch_strided(c->prec, a, c->segY, c->n1, v, -1, c->tw1, bt, ny, c->nt_y, c->stream);
+    if (which == 1) return launch_y(c, a, v, -1, bt, ny);
     return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, zl_own(c), v, bt, ny,
                         c->twz, c->twp, c->pz, false, c->stream);
   };
@@ -982,7 +995,7 @@ int autotune(xpfc_ctx* c) {
   // order of preference (wide tiles write whole 128-byte lines); a later
   // variant must be 3 % faster to replace an earlier one.
   const std::vector<int> xs{16, 8, 4};
-  const std::vector<int> ys{16, 8, 4};
+  const std::vector<int> ys{16, 16 | YRUNS_PAIRED, 8, 8 | YRUNS_PAIRED, 4, 4 | YRUNS_PAIRED};
   const std::vector<int> zs{256, 256 | ZFREE_REGS, 128, 128 | ZFREE_REGS, 512, 512 | ZFREE_REGS,
                             64, 64 | ZFREE_REGS};
   for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
@@ -1048,7 +1061,7 @@ extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
   const long long nl = (long long)c->n0 * c->n1;
   if (v[0] > 0 && strided_blocks(c->prec, c->n0, v[0], c->segX) < 0)
     return fail(XPFC_ERR_ARG, "tile width %d not available for n0 = %d", v[0], c->n0);
-  if (c->n1 > 1 && v[1] > 0 && strided_blocks(c->prec, c->n1, v[1], c->segY) < 0)
+  if (c->n1 > 1 && v[1] > 0 && strided_blocks(c->prec, c->n1, v[1] & ~YRUNS_PAIRED, c->segY) < 0)
     return fail(XPFC_ERR_ARG, "tile width %d not available for n1 = %d", v[1], c->n1);
   if (v[2] > 0 && zpass_blocks(c->prec, c->H, v[2], nl) < 0)
     return fail(XPFC_ERR_ARG, "real-space block %d not available for n2 = %d", v[2], c->n2);

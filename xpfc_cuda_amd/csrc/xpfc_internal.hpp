@@ -21,6 +21,7 @@ struct Seg {
 struct Segs {
   Seg s[2];
   int tiles0, tiles1;
+  int lgc;  // workgroup order over the XCDs: runs of 2^lgc tiles, or < 0: one run per XCD
 };
 
 // Wavenumber geometry of the packed spectrum.  The main block is laid out
@@ -92,6 +93,9 @@ struct Batch {
 // Register-allocation variant flag (the arithmetic is identical), or-ed into
 // the real-space workgroup size: no occupancy hint.
 constexpr int ZFREE_REGS = 1;
+// Bit 0 of an axis-1 pass variant (tile width 4, 8 or 16): workgroups dealt to
+// the XCDs in runs of two tiles instead of one contiguous run per XCD.
+constexpr int YRUNS_PAIRED = 1;
 
 // Per-workgroup FP64 partial sums.  X pass: {sum w c2 |psi_k|^2, sum w |R_k|^2}.
 // Z pass: {sum psi, sum psi^2, sum psi^3, sum psi^4}.

@@ -38,12 +38,20 @@ template <typename T, int L, int TC> constexpr bool tile_ok() {
 }
 
 // XCD-aware block order: the dispatcher deals consecutive workgroups to the 8
-// XCDs round-robin; remap so that each XCD receives a contiguous run of tiles
-// and neighbouring column tiles share one L2.
-__device__ __forceinline__ int xcd_tile(int b, int n) {
+// XCDs round-robin.  lgc < 0, or fewer tiles than one round: each XCD receives
+// one contiguous run of tiles.  Otherwise each XCD receives runs of 2^lgc
+// neighbouring tiles (column tiles that share 128-byte lines share one L2),
+// the runs dealt to the XCDs in turn, so that the eight XCDs work on nearby
+// columns at any time; tiles past the last whole round keep their order.
+__device__ __forceinline__ int xcd_tile(int b, int n, int lgc) {
   constexpr int X = 8;
-  const int per = n / X, rem = n % X;
   const int xcd = b % X, slot = b / X;
+  if (lgc >= 0 && n >= (X << lgc)) {
+    const int full = (n >> (lgc + 3)) << (lgc + 3);
+    if (b >= full) return b;
+    return ((((slot >> lgc) * X) + xcd) << lgc) + (slot & ((1 << lgc) - 1));
+  }
+  const int per = n / X, rem = n % X;
   return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
@@ -84,7 +92,7 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   __shared__ C twl[L];
   buf += (long long)bt.slots[blockIdx.y] * bt.stride;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
-  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
+  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ci, TC);
   const int rs = loc.sg->rs;
   C* p = buf + loc.off;
   C v[E];
@@ -127,7 +135,7 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   partial += (long long)slot * bt.pstride;
   const Phys& ph = bt.ia[slot].ph;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
-  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x), ci, TC);
+  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ci, TC);
   const int rs = loc.sg->rs;
   const bool act = loc.act;
   C* pw = work + loc.off;
