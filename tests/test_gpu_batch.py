@@ -124,3 +124,28 @@ def test_active_subset_and_slot_reuse(prec):
     assert np.array_equal(f2, a2)
     _same_samples(s2, smp2)
 
+
+
+@pytest.mark.parametrize("yv", [16, 17])
+def test_large_batch_streaming_passes_equal_alone(yv):
+    """A batch whose spectra exceed the 256 MiB last-level cache runs its
+    axis-1 passes with streaming cache hints; 32 instances of 128^3 FP32 are
+    that large, a single one is not.  Hints change no arithmetic: every
+    instance must match its run alone bitwise (yv 17: the axis-1 workgroups
+    dealt to the XCDs in pairs)."""
+    from xpfc_cuda_amd import instance_bytes
+    shape, K = (128, 128, 128), 32
+    _flush("large batch %s x %d yv=%d" % (shape, K, yv))
+    _, p = growth_case(shape, prec=XPFC_FP32_MIXED, t_sample=0)
+    assert K * instance_bytes(p) // 3 >= 256 << 20 > instance_bytes(p) // 3
+    psis = _fields(shape, 3, seed=21)
+    v = (8, yv, 256)
+    with BatchContext([p] * K) as bc:
+        bc.set_variants(*v)
+        for s in range(K):
+            bc.set_field(s, psis[s % 3])
+        bc.step(6)
+        got = [bc.get_field(s) for s in (0, 1, 2, K - 1)]
+    for s, f in zip((0, 1, 2, K - 1), got):
+        a, _ = _alone(p, psis[s % 3], 6, v)
+        assert np.array_equal(f, a), "instance %d differs" % s

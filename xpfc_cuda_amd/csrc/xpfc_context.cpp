@@ -480,12 +480,15 @@ extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
 // ---------------------------------------------------------------------------
 namespace {
 
-// axis-1 pass with variant v: tile width and workgroup order
+// axis-1 pass with variant v: tile width and workgroup order.  The streaming
+// hints apply to tiles whose rows are whole 128-byte lines (measured at 512^3
+// FP32: with 32- and 64-byte rows they double the pass's HBM traffic).
 int launch_y(xpfc_ctx* c, void* spec, int v, int dir, const Batch& bt, int ny) {
   Segs sg = c->segY;
   sg.lgc = (v & YRUNS_PAIRED) ? XCD_RUN_Y : -1;
-  return launch_strided(c->prec, spec, sg, c->n1, v & ~YRUNS_PAIRED, dir, c->tw1, bt, ny, c->nt_y,
-                        c->stream);
+  const int tile = v & ~YRUNS_PAIRED;
+  const bool nt = c->nt_y && (size_t)tile * c->csize >= 128;
+  return launch_strided(c->prec, spec, sg, c->n1, tile, dir, c->tw1, bt, ny, nt, c->stream);
 }
 
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
