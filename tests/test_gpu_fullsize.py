@@ -1,0 +1,89 @@
+"""The 512^3 workload at full size, through properties that do not need the
+oracle (which is far too slow at this size): the closed-form evolution of a
+linear two-mode field, and bitwise agreement of different pass variants and of
+graph replays with eager launches on the nonlinear workload, with the mass
+mode unchanged."""
+import math
+
+import numpy as np
+import pytest
+
+from xpfc_cuda_amd import Context, Params, XPFC_DCF_MAX, XPFC_FP32_MIXED, XPFC_FP64
+
+pytestmark = pytest.mark.gpu
+
+N = 512
+L = 64.0
+TWO_PI = 2.0 * math.pi
+PEAKS = ((TWO_PI, 1.0, 1.0), (TWO_PI * 2.0 / math.sqrt(3.0), 1.0, 1.0))  # fcc
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _params(prec, sigma, a, b, t_sample=0):
+    return Params(n=(N, N, N), L=(L, L, L), dt=0.5, Mo=1.0, eps=0.0, a=a, b=b, sigma=sigma,
+                  psi0=0.05, peaks=PEAKS, dcf_mode=XPFC_DCF_MAX, precision=prec,
+                  t_sample=t_sample)
+
+
+def _plane_wave(idx, amp, out):
+    """out += amp * cos(k . x) with k = 2 pi / L * idx on the 512^3 grid."""
+    x = np.arange(N) * (L / N)
+    kx, ky, kz = (TWO_PI / L * i for i in idx)
+    ph = (kx * x)[:, None, None] + (ky * x)[None, :, None] + (kz * x)[None, None, :]
+    np.cos(ph, out=ph)
+    ph *= amp
+    out += ph
+
+
+@pytest.mark.parametrize("prec,tol", [(XPFC_FP64, 1e-13), (XPFC_FP32_MIXED, 1e-7)])
+def test_linear_two_modes_closed_form_512cube(prec, tol):
+    """a = b = 0 (no nonlinearity), sigma = 0: each Fourier mode is multiplied
+    by 1 / (1 + dt M k^2 (1 - C2(k))) per step, C2(k) = max_i exp(-(|k| - k_i)^2
+    / (2 alpha_i^2)).  One mode far below the peaks, one close to the first."""
+    _flush("512^3 linear closed form prec=%d" % prec)
+    p = _params(prec, sigma=0.0, a=0.0, b=0.0)
+    modes = [((3, 2, 1), 0.01), ((48, 32, 24), 0.008)]
+    psi = np.full((N, N, N), 0.05)
+    for idx, amp in modes:
+        _plane_wave(idx, amp, psi)
+    mean0 = float(np.mean(psi))  # 0.05 up to the rounding of the cosines
+    nsteps = 10
+    with Context(p) as c:
+        c.set_field(psi)
+        c.step(nsteps)
+        got = c.get_field()
+        mean = c.avg_calc()
+    expect = np.full((N, N, N), 0.05)
+    for idx, amp in modes:
+        k = TWO_PI / L * math.sqrt(sum(i * i for i in idx))
+        c2 = max(math.exp(-(k - kp) ** 2 / (2.0 * al * al)) for kp, al, _ in PEAKS)
+        g = 1.0 / (1.0 + 0.5 * k * k * (1.0 - c2))
+        assert 0.1 < g ** nsteps < 0.9  # both modes measurably damped
+        _plane_wave(idx, amp * g ** nsteps, expect)
+    assert np.max(np.abs(got - expect)) <= tol
+    # the mass mode is carried unchanged; the library and numpy sum the 1.3e8
+    # cells in different orders (FP64 rounding of order 1e-13 relative)
+    assert mean == pytest.approx(mean0, abs=1e-13)
+
+
+def test_variants_and_graphs_bitwise_512cube():
+    """The nonlinear fcc workload (FP32-mixed): two different variant sets, one
+    run from graph replays and one launched eagerly, give the same field bit
+    for bit after 12 steps; the mean stays at n0."""
+    _flush("512^3 variants / graphs bitwise")
+    p = _params(XPFC_FP32_MIXED, sigma=0.1, a=1.0, b=1.0 / 3.0)
+    out = []
+    for v, graphs in [((8, 17, 256), True), ((16, 16, 129), False)]:
+        with Context(p) as c:
+            c.set_variants(*v)
+            c.set_graphs(graphs)
+            c.initialize_noise(0.01, 2)
+            c.step(12)
+            out.append(c.get_field())
+            assert c.avg_calc() == pytest.approx(0.05, abs=1e-15)
+            F, _ = c.fe_calc()
+            assert math.isfinite(F)
+    assert np.array_equal(out[0], out[1])
