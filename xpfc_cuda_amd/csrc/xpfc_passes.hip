@@ -236,8 +236,10 @@ template <typename T, int H> constexpr int z_elems() {
   return H < 8 ? H : ((H >= 128 && sizeof(T) == 4) ? 16 : 8);
 }
 template <typename T, int H> constexpr int z_tpl() { return H / z_elems<T, H>(); }  // threads per line
+// (a line's lanes must lie in one wavefront: the twiddle pairs are exchanged
+// with __shfl; with n2 <= 1024 every supported H = n2/2 satisfies it)
 template <typename T, int H, int NTZ> constexpr bool z_ok() {
-  return NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0 &&
+  return z_tpl<T, H>() <= 64 && NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0 &&
          ((long long)(NTZ / z_tpl<T, H>()) * (H + 1) + 2 * H) * (long long)sizeof(cpx<T>) <= 152 * 1024;
 }
 // LDS line of H rows plus the X[H] slot.  Rows are XOR-swizzled within each
@@ -253,8 +255,9 @@ template <int H, int E> struct ZIdx {
   }
 };
 
-// The C2R pre-twiddle needs X[k] and X[H-k]: the mirror element is re-read
-// from global memory (a cache hit: this workgroup is loading that line).
+// The C2R pre-twiddle and the R2C post-twiddle pair bin k with bin H-k; the
+// mirror is exchanged with __shfl among the TPL lanes that hold the line (a
+// power of two <= 64, so a line never straddles two wavefronts).
 // FREE: no occupancy hint (the compiler may use more registers instead of
 // spilling); a register-allocation variant only, the arithmetic is identical.
 template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE>
@@ -276,6 +279,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   const ZArgs za = bt.ia[slot].za;
   constexpr int E = z_elems<T, H>();
   constexpr int TPL = z_tpl<T, H>();
+  static_assert(TPL <= 64 && (TPL & (TPL - 1)) == 0, "a line must lie in one wavefront");
   constexpr int G = NTZ / TPL;
   __shared__ C lds[G * z_len<H>()];
   __shared__ C twl[H], twpl[H];  // the H-point FFT twiddles and the half-spectrum twiddles
@@ -372,19 +376,23 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
     }
   }
 
-  // forward H-point transform and the real-to-half-spectrum post twiddle
+  // forward H-point transform and the real-to-half-spectrum post twiddle,
+  // which pairs Z[k] with Z[H-k]: the mirror bin is exchanged across the
+  // line's lanes as in the pre-twiddle above (element E-1-m of lane TPL-t;
+  // for t = 0 this lane's element E-m, or Z[0] itself)
   fft_line<T, H, E, -1>(v, t, lds, idx, twl);
-  __syncthreads();
-#pragma unroll
-  for (int m = 0; m < E; ++m) lds[idx(t + m * TPL)] = v[m];
-  __syncthreads();
   C* op = reinterpret_cast<C*>(out) + off_out;
   const T half = (T)0.5;
+  const int mlane = threadIdx.x & 63;
+  const int msrc = mlane - t + ((TPL - t) & (TPL - 1));
 #pragma unroll
   for (int m = 0; m < E; ++m) {
     const int k = t + m * TPL;
-    const C zk = lds[idx(k)];
-    const C zm = lds[idx((H - k) & (H - 1))];
+    const C zk = v[m];
+    C zm;
+    zm.x = __shfl(v[E - 1 - m].x, msrc, 64);
+    zm.y = __shfl(v[E - 1 - m].y, msrc, 64);
+    if (t == 0) zm = v[(E - m) & (E - 1)];
     const C e = mk<C>(half * (zk.x + zm.x), half * (zk.y - zm.y));
     const C o = mk<C>(half * (zk.y + zm.y), -half * (zk.x - zm.x));
     if (act) {
@@ -656,11 +664,15 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
     // set-up and read-back modes: one workgroup shape (256 threads)
     XPFC_LSWITCH(H, {
       constexpr int NZ = 256;
-      switch (mode) {
-        case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
-        case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
-        case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s); break;
-        default: return -1;
+      if constexpr (z_ok<T, LL, NZ>()) {
+        switch (mode) {
+          case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
+          case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
+          case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s); break;
+          default: return -1;
+        }
+      } else {
+        return -3;
       }
     });
   }
