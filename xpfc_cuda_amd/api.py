@@ -426,6 +426,9 @@ class Context:
         return tuple(v)
 
     def set_variants(self, tc_x: int = 0, tc_y: int = 0, ntz: int = 0):
+        """Pin the pass variants (0 = keep): axis-0 tile width (4/8/16); axis-1
+        tile width (+1: workgroups dealt to the XCDs in pairs); real-space
+        workgroup size 64..512 (+1: no occupancy hint)."""
         v = (ct.c_int * 3)(tc_x, tc_y, ntz)
         _check(lib().xpfc_set_variants(self.handle, v), "xpfc_set_variants")
 

@@ -20,9 +20,11 @@
 //
 // Variants: the strided kernels take the tile width TC (columns per
 // workgroup) and the real-space kernel its workgroup size NTZ as template
-// parameters.  They change only how lines are grouped into workgroups, never
-// the arithmetic of a line, so every variant gives bitwise-identical fields;
-// the autotuner picks the fastest per size.
+// parameters; the order in which workgroups are dealt to the XCDs and the
+// streaming cache hints of the axis-1 passes are further options.  They change
+// only how lines are grouped, ordered and cached, never the arithmetic of a
+// line, so every variant gives bitwise-identical fields; the autotuner picks
+// the fastest per size.
 #include "xpfc_device.hpp"
 
 namespace xpfc {
