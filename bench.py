@@ -198,11 +198,12 @@ def batched_run(steps=2000, ninst=64, seed=0):
     return res
 
 
-def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 << 25):
+def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 << 25,
+                lookahead=1):
     """Poisson job stream through the continuous-batching service."""
     from xpfc_cuda_amd import serving as S
     jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d)
-    svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells))
+    svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells), lookahead=lookahead)
     try:
         svc.warm([(2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128), (3, 256)])
         rep = svc.run(jobs, realtime=True,
@@ -211,7 +212,7 @@ def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 
         offered = sum(j.cells * j.steps for j in jobs) / seconds
         rep.update({"arrival_rate": rate, "window_s": seconds, "frac_3d": frac_3d,
                     "offered_cell_updates_per_s": offered,
-                    "budget_gb": budget / 1e9,
+                    "budget_gb": budget / 1e9, "lookahead_blocks": lookahead,
                     "rounding": "side log-uniform, rounded to the nearest power of two in log2",
                     "steps": "U{500..5000}", "sigma": "U[0.05,0.2]", "n0": "U[0.02,0.08]"})
     finally:
@@ -234,6 +235,8 @@ def main():
     ap.add_argument("--serve-3d", type=float, default=0.5, help="fraction of 3-D jobs")
     ap.add_argument("--serve-class-cells", type=int, default=1 << 25,
                     help="slot policy: cells per size-class launch")
+    ap.add_argument("--serve-lookahead", type=int, default=1,
+                    help="blocks handed at once to a size class no queued job waits for")
     ap.add_argument("--only", choices=["batch", "serve", "configs"],
                     help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -264,7 +267,8 @@ def main():
         return
     if args.only == "serve":
         print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d,
-                                                  class_cells=args.serve_class_cells)}), flush=True)
+                                                  class_cells=args.serve_class_cells,
+                                                  lookahead=args.serve_lookahead)}), flush=True)
         return
     n = args.n
     ms, wall, passes, samples, variants, io = gpu_run(n, 1, args.steps, args.warmup)
@@ -331,7 +335,8 @@ def main():
         out["batched"] = batched_run()
     if args.serve > 0:
         out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d,
-                                     class_cells=args.serve_class_cells)
+                                     class_cells=args.serve_class_cells,
+                                     lookahead=args.serve_lookahead)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

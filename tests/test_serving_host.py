@@ -37,7 +37,8 @@ class FakeCtx:
         FakeCtx.log.append(("init", s, seed))
 
     def step(self, n):
-        assert self.busy_polls == 0 and 1 <= n <= S.BLOCK
+        assert self.busy_polls == 0 and 1 <= n <= S.BLOCK * 8
+        FakeCtx.log.append(("step", n))
         for s in self.active:
             self.steps[s] += n
         self.busy_polls = 2
@@ -157,3 +158,17 @@ def test_diverged_job_stops_early_and_frees_its_slot(fake, monkeypatch):
     # job 2 took job 1's slot long before job 1's 1000 steps would have ended
     assert by[2].admitted >= by[1].finished
     assert by[2].finished < by[0].finished
+
+
+def test_blocks_lengthen_only_while_no_job_waits(fake):
+    """While a job waits for the class's only slot, blocks stay at BLOCK
+    steps; once nothing waits, LOOKAHEAD blocks are handed over at once."""
+    jobs = [_job(0, 64, 130), _job(1, 64, 470)]
+    svc = S.Service(budget_bytes=1e12, slots=lambda d, s: 1, lookahead=4)
+    svc.run(jobs, realtime=False, idle_sleep=0)
+    steps = [e[1] for e in FakeCtx.log if e[0] == "step"]
+    # job 0 runs while job 1 waits: 50, 50, 30; then job 1 alone: 200, 200, 70
+    assert steps == [50, 50, 30, 200, 200, 70]
+    assert {j.jid: j.F_ref for j in svc.done} == {0: 130, 1: 470}
+    with pytest.raises(ValueError):
+        S.Service(lookahead=0)
