@@ -76,6 +76,26 @@ def gpu_run(n, prec, steps, warmup, seed=2):
     return ms, wall, passes, samples, variants, io
 
 
+def accuracy_run(n, steps=1000, seed=2):
+    """FP32-mixed against the FP64 path on the same workload after `steps`
+    steps: max |dn| and the relative free-energy difference (the FP64 path is
+    parity-tested against the FP64 oracle; the oracle itself is too slow at
+    this size)."""
+    import numpy as np
+    import xpfc_cuda_amd as X
+    fields, fe = [], []
+    for prec in (1, 0):
+        with X.Context(fcc_params(n, prec, t_sample=0)) as ctx:
+            ctx.initialize_noise(0.01, seed)
+            ctx.step(steps)
+            fields.append(ctx.get_field())
+            fe.append(ctx.fe_calc()[0])
+    dn = float(np.max(np.abs(fields[0] - fields[1])))
+    return {"steps": steps, "max_abs_dn": dn, "fe_rel": abs(fe[0] - fe[1]) / abs(fe[1]),
+            "F_fp32": fe[0], "F_fp64": fe[1],
+            "reference": "FP64 GPU path on the same initial field"}
+
+
 def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
     """The FP64 oracle (numpy/scipy pocketfft, all host threads) on a bounded
     number of steps of the same 512^3 workload; steps/s."""
@@ -313,6 +333,7 @@ def main():
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
         out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
+        out["accuracy_vs_fp64"] = accuracy_run(n)
     out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1] & ~1,
                        "axis1_xcd_pairs": bool(variants[1] & 1),
                        "real_space_block": variants[2] & ~1,

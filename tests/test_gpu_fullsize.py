@@ -87,3 +87,23 @@ def test_variants_and_graphs_bitwise_512cube():
             F, _ = c.fe_calc()
             assert math.isfinite(F)
     assert np.array_equal(out[0], out[1])
+
+
+def test_fp32_mixed_vs_fp64_1000_steps_512cube():
+    """The 512^3 fcc workload (sigma = 0.1, n0 = 0.05 + 0.01 N(0,1)) after
+    1000 steps in FP32-mixed and in FP64 from the same initial field: max |dn|
+    <= 1e-5 and relative free-energy difference <= 1e-7.  (The FP64 path is
+    held to the FP64 oracle at smaller sizes; the oracle itself would take
+    about 25 CPU-minutes here.)"""
+    _flush("512^3 FP32-mixed vs FP64, 1000 steps")
+    fields, fe = [], []
+    for prec in (XPFC_FP32_MIXED, XPFC_FP64):
+        with Context(_params(prec, sigma=0.1, a=1.0, b=1.0 / 3.0)) as c:
+            c.initialize_noise(0.01, 2)
+            c.step(1000)
+            fields.append(c.get_field())
+            fe.append(c.fe_calc()[0])
+    dn = float(np.max(np.abs(fields[0] - fields[1])))
+    print("max|dn| = %.3e, FE rel = %.3e" % (dn, abs(fe[0] - fe[1]) / abs(fe[1])), flush=True)
+    assert dn <= 1e-5
+    assert abs(fe[0] - fe[1]) <= 1e-7 * abs(fe[1])

@@ -31,10 +31,13 @@ def _jobs():
     return jobs
 
 
-def test_alone_equals_mix():
+@pytest.mark.parametrize("lookahead", [1, 4])
+def test_alone_equals_mix(lookahead):
+    print("serving lookahead=%d" % lookahead, flush=True)
     jobs = _jobs()
     budget = 4 * max(S.instance_bytes(S.job_params(j)) for j in jobs)
-    svc = S.Service(budget_bytes=budget, slots=lambda d, s: 2, keep_fields=True)
+    svc = S.Service(budget_bytes=budget, slots=lambda d, s: 2, keep_fields=True,
+                    lookahead=lookahead)
     try:
         rep = svc.run(jobs, realtime=False, idle_sleep=0.0)
     finally:
