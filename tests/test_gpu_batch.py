@@ -149,3 +149,20 @@ def test_large_batch_streaming_passes_equal_alone(yv):
     for s, f in zip((0, 1, 2, K - 1), got):
         a, _ = _alone(p, psis[s % 3], 6, v)
         assert np.array_equal(f, a), "instance %d differs" % s
+
+
+def test_batch_larger_than_the_device_fails_cleanly():
+    """A batch whose buffers cannot be allocated (512^3 FP64 x 1000 instances,
+    about 1 TB per buffer) fails with XPFC_ERR_NOMEM, and the device stays
+    usable for the next context."""
+    from xpfc_cuda_amd import XPFCError, api
+    _flush("oversized batch")
+    _, p = make((512, 512, 512), prec=XPFC_FP64)
+    with pytest.raises(XPFCError) as e:
+        BatchContext([p] * 1000)
+    assert e.value.status == api.XPFC_ERR_NOMEM
+    _, q = growth_case((32, 32), t_sample=0)
+    with Context(q) as c:
+        c.initialize_noise(0.01, 1)
+        c.step(5)
+        assert np.all(np.isfinite(c.get_field()))

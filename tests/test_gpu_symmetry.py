@@ -114,3 +114,22 @@ def test_fft_vs_second_library(shape):
         got = c.fft_r2c(x)
     assert got.shape == ref.shape
     assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)) * 4
+
+
+@pytest.mark.parametrize("shape,mode", [((64, 32), (5, 3)), ((16, 32, 64), (3, 7, 9)),
+                                        ((128, 128, 128), (17, 40, 21))])
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+def test_fft_plane_wave_known_answer(shape, mode, prec):
+    """R2C of cos(2 pi m.x / n) is N/2 in bin m of the half spectrum and zero
+    elsewhere (the conjugate bin -m lies in the unstored half)."""
+    _flush("plane wave %s %s prec=%d" % (shape, mode, prec))
+    _, p = make(shape, prec=prec)
+    grids = np.meshgrid(*[np.arange(n) / n for n in shape], indexing="ij")
+    x = np.cos(2.0 * np.pi * sum(m * g for m, g in zip(mode, grids)))
+    with Context(p) as c:
+        got = c.fft_r2c(x)
+    N = x.size
+    expect = np.zeros_like(got)
+    expect[mode] = N / 2.0
+    tol = (1e-12 if prec == XPFC_FP64 else 2e-6) * N
+    assert np.max(np.abs(got - expect)) <= tol

@@ -312,6 +312,12 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
   const size_t sbytes = (size_t)K * c->stride * c->csize;
+  // the runtime may grant allocations beyond the device's memory and fail
+  // later; refuse what cannot fit up front
+  size_t mfree = 0, mtotal = 0;
+  if (hipMemGetInfo(&mfree, &mtotal) == hipSuccess && sbytes > mfree / 3)
+    return bail(fail(XPFC_ERR_NOMEM, "%d instance(s) need 3 x %zu bytes of device memory, %zu free", K,
+                     sbytes, mfree));
   if (hipMalloc(&c->psi, sbytes) != hipSuccess || hipMalloc(&c->work, sbytes) != hipSuccess ||
       hipMalloc(&c->aux, sbytes) != hipSuccess)
     return bail(fail(XPFC_ERR_NOMEM, "device allocation of %zu bytes x3 failed", sbytes));

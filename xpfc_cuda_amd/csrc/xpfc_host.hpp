@@ -34,9 +34,9 @@ double c2hat_host(double kmag, const xpfc_params& p);
   do {                                                                                      \
     int r_ = (expr);                                                                        \
     if (r_ < 0) {                                                                           \
-      hipError_t e_ = hipGetLastError();                                                    \
       return xpfc::fail(XPFC_ERR_DEVICE, "%s failed (%d): %s", #expr, r_,                   \
-                        hipGetErrorString(e_));                                             \
+                        r_ == -2 ? hipGetErrorString(xpfc::launch_error())                  \
+                                 : "variant or size not available");                        \
     }                                                                                       \
   } while (0)
 

@@ -108,6 +108,12 @@ constexpr int NSUMS = 6;
 // workgroup size (64..512) of the real-space pass; a combination a size does
 // not support returns -3.  launch_xstep returns its workgroup count (= number
 // of X partials) on success.
+// Launchers return launch_status(v): v when the launches just made succeeded,
+// else -2, with the HIP error kept for the caller's message (launch_error(),
+// per host thread).
+int launch_status(int ok_value = 0);
+hipError_t launch_error();
+
 // `ny` is the number of instances covered (entries of bt.slots used).  `nt`
 // selects the streaming-hint loads and stores of the strided pass.
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,

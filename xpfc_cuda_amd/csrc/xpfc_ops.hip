@@ -189,7 +189,7 @@ int launch_noise_sum(long long n, unsigned long long seed, double* partial, int*
   *nblocks = nb;
   hipLaunchKernelGGL(k_noise_sum, dim3(nb), dim3(256), 0, s, n, (uint32_t)seed, (uint32_t)(seed >> 32),
                      partial);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_noise_fill(int prec, void* out, long long n, long long plane, long long pitch,
@@ -201,7 +201,7 @@ int launch_noise_fill(int prec, void* out, long long n, long long plane, long lo
   else
     hipLaunchKernelGGL(k_noise_fill<double>, dim3(nb), dim3(256), 0, s, (double*)out, n, plane, pitch,
                        (uint32_t)seed, (uint32_t)(seed >> 32), amp, mean_z);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 // Divergence probe: one workgroup; flag[s] = 1 when two modes of instance s
@@ -225,7 +225,7 @@ int launch_probe(int prec, const void* psi, long long stride, long long iny, int
   else
     hipLaunchKernelGGL(k_probe<double>, dim3(1), dim3(256), 0, s, (const cpx<double>*)psi, stride,
                        iny, K, flag);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s) {
@@ -233,7 +233,7 @@ int launch_power(int prec, const void* src, void* dst, long long n, int p, hipSt
     hipLaunchKernelGGL(k_power<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, (float*)dst, n, p);
   else
     hipLaunchKernelGGL(k_power<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, (double*)dst, n, p);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_scale(int prec, void* buf, long long n, double f, hipStream_t s) {
@@ -241,7 +241,7 @@ int launch_scale(int prec, void* buf, long long n, double f, hipStream_t s) {
     hipLaunchKernelGGL(k_scale<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (float*)buf, n, (float)f);
   else
     hipLaunchKernelGGL(k_scale<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (double*)buf, n, f);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 static long long natural_count(const KGeom& g, int n2) {
@@ -257,7 +257,7 @@ int launch_convolve(int prec, const void* psi_k, void* out, const KGeom& g, cons
   else
     hipLaunchKernelGGL(k_convolve<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        (const cpx<double>*)psi_k, (cpx<double>*)out, n, n2 / 2 + 1, g, ph);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_euler(int prec, const void* psi_k, const void* p2_k, const void* p3_k, void* out,
@@ -271,7 +271,7 @@ int launch_euler(int prec, const void* psi_k, const void* p2_k, const void* p3_k
     hipLaunchKernelGGL(k_euler<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        (const cpx<double>*)psi_k, (const cpx<double>*)p2_k, (const cpx<double>*)p3_k,
                        (cpx<double>*)out, n, n2 / 2 + 1, g, ph, a, b);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void* p2_k,
@@ -286,7 +286,7 @@ int launch_residual(int prec, const void* psi_kp1, const void* psi_k, const void
     hipLaunchKernelGGL(k_residual<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        (const cpx<double>*)psi_kp1, (const cpx<double>*)psi_k, (const cpx<double>*)p2_k,
                        (const cpx<double>*)p3_k, (cpx<double>*)out, n, n2 / 2 + 1, g, ph, a, b);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, int n1, long long xs,
@@ -298,7 +298,7 @@ int launch_pack(int prec, const void* natural, void* main_ny, int n0n1, int H, i
   else
     hipLaunchKernelGGL(k_pack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        (const cpx<double>*)natural, (cpx<double>*)main_ny, n0n1, H, n1, xs);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H, int n1, long long xs,
@@ -310,7 +310,7 @@ int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H,
   else
     hipLaunchKernelGGL(k_unpack<double>, dim3(grid_for(n, 256)), dim3(256), 0, s,
                        (const cpx<double>*)main_ny, (cpx<double>*)natural, n0n1, H, n1, xs);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s) {
@@ -318,7 +318,7 @@ int launch_to_double(int prec, const void* src, double* dst, long long n, hipStr
     hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, dst, n);
   else
     hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, dst, n);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s) {
@@ -326,7 +326,7 @@ int launch_from_double(int prec, const double* src, void* dst, long long n, hipS
     hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_for(n, 256)), dim3(256), 0, s, src, (float*)dst, n);
   else
     hipLaunchKernelGGL((k_convert<double, double>), dim3(grid_for(n, 256)), dim3(256), 0, s, src, (double*)dst, n);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 }  // namespace xpfc

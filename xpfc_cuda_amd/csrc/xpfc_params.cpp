@@ -27,6 +27,10 @@ int fail(int code, const char* fmt, ...) {
   vsnprintf(buf, sizeof(buf), fmt, ap);
   va_end(ap);
   g_last_error = buf;
+  // a failed runtime call (for example an allocation refused before a
+  // context is torn down) leaves its error pending; clear it so the next
+  // launch check does not report it
+  (void)hipGetLastError();
   return code;
 }
 

@@ -480,6 +480,19 @@ __global__ __launch_bounds__(256) void k_kspace_sums(const cpx<T>* __restrict__ 
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
+namespace {
+thread_local hipError_t t_launch_error = hipSuccess;
+}
+
+int launch_status(int ok_value) {
+  const hipError_t e = hipGetLastError();
+  if (e == hipSuccess) return ok_value;
+  t_launch_error = e;
+  return -2;
+}
+
+hipError_t launch_error() { return t_launch_error; }
+
 #define XPFC_LSWITCH(L, ...)                                    \
   switch (L) {                                                  \
     case 4: { constexpr int LL = 4; __VA_ARGS__; } break;       \
@@ -546,7 +559,7 @@ static int strided_T(void* buf, const Segs& segs0, int L, int tc, int dir, const
       return -3;
     }
   }));
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
@@ -602,7 +615,7 @@ static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, cons
       return -3;
     }
   }));
-  return hipGetLastError() == hipSuccess ? nb : -2;
+  return launch_status(nb);
 }
 
 int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
@@ -678,7 +691,7 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
       }
     });
   }
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
@@ -693,7 +706,7 @@ int launch_finalize(const double* px, int nx, const double* pz, int nz, const Ba
                     double* out, hipStream_t s) {
   if (ny == 0) return 0;
   hipLaunchKernelGGL(k_finalize, dim3(ny), dim3(1024), 0, s, px, nx, pz, nz, bt, out);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny, const KGeom& g,
@@ -708,7 +721,7 @@ int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny
   else
     hipLaunchKernelGGL(k_kspace_sums<double>, dim3(nb), dim3(256), 0, s, (const cpx<double>*)psi,
                        nmain, nny, g, bt, partial);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 // C2 table of one instance over the integer m = |k|^2 / dksq
@@ -721,7 +734,7 @@ __global__ __launch_bounds__(256) void k_c2table(double* __restrict__ tab, int m
 int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s) {
   if (mcut <= 0) return 0;
   hipLaunchKernelGGL(k_c2table, dim3((mcut + 255) / 256), dim3(256), 0, s, tab, mcut, dksq, ph);
-  return hipGetLastError() == hipSuccess ? 0 : -2;
+  return launch_status();
 }
 
 }  // namespace xpfc
