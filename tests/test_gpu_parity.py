@@ -118,8 +118,10 @@ def test_c1_config_1000_steps(prec):
 
 
 @pytest.mark.parametrize("prec", PRECS)
-def test_operator_kernels_vs_oracle(prec):
-    op, lp = growth_case((16, 32), prec=prec)
+@pytest.mark.parametrize("shape", [(16, 32), (8, 16, 32), (16, 16, 16)])
+def test_operator_kernels_vs_oracle(prec, shape):
+    _flush("operators %s prec=%d" % (shape, prec))
+    op, lp = growth_case(shape, prec=prec)
     rng = np.random.default_rng(11)
     psi = 0.05 + 0.02 * rng.standard_normal(op.n)
     h = op.n[-1] // 2 + 1
