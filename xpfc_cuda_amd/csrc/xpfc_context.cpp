@@ -41,9 +41,11 @@ constexpr size_t STREAM_BYTES = (size_t)256 << 20;
 
 // Workgroup order over the XCDs (log2 of the runs of tiles dealt to each XCD
 // in turn).  Measured against one contiguous run per XCD: the axis-0 pass with
-// runs of 64 tiles -4 % at 512^3 FP32 and FP64, -6 % at 256^3 FP64; the
-// axis-1 passes with runs of two -6 % at 512^3 FP32 but +3 % at 512^3 FP64 and
-// +5 % at 256^3 FP64, so that order is a tuned variant (YRUNS_PAIRED).
+// runs of 64 tiles -4 % at 512^3 FP32 and FP64, -6 % at 256^3 FP64, and with
+// runs of 8 tiles -10 % on batches of 2-D 1024^2 fields (129 tiles each; no
+// change at 512^2 and 256^2); the axis-1 passes with runs of two -6 % at 512^3
+// FP32 but +3 % at 512^3 FP64 and +5 % at 256^3 FP64, so that order is a
+// tuned variant (YRUNS_PAIRED).
 constexpr int XCD_RUN_X = 6, XCD_RUN_Y = 1;
 
 long long plane_stride(int n1, int H, size_t csize) {
@@ -331,7 +333,7 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   c->segX.s[1] = Seg{c->nyoff, 0, c->n1, 1, c->n1, 1, 0, 1};
   c->segY.s[0] = Seg{0, c->xs, c->H, 1, c->H, c->n0, 0, 0};
   c->segY.s[1] = Seg{c->nyoff, 0, 1, c->n1, c->n0, 1, 0, 1};
-  c->segX.lgc = XCD_RUN_X;
+  c->segX.lgc = -1;  // set per launch from the tile count (x_segs)
   c->segY.lgc = -1;  // set per launch from the axis-1 variant
   c->tc_x = default_tc(c->prec, c->n0, c->segX);
   c->tc_y = c->n1 > 1 ? default_tc(c->prec, c->n1, c->segY) : 0;
@@ -486,6 +488,18 @@ extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
 // ---------------------------------------------------------------------------
 namespace {
 
+// Axis-0 segments for tile width tc: runs of up to 2^XCD_RUN_X tiles, at
+// least two rounds of them over the 8 XCDs; with fewer than 64 tiles (runs of
+// 4), one contiguous run per XCD.
+Segs x_segs(const xpfc_ctx* c, int tc) {
+  Segs sg = c->segX;
+  const int n = strided_blocks(c->prec, c->n0, tc, c->segX);
+  int lg = XCD_RUN_X;
+  while (lg >= 0 && n < (16 << lg)) --lg;
+  sg.lgc = lg >= 2 ? lg : -1;
+  return sg;
+}
+
 // axis-1 pass with variant v: tile width and workgroup order.  The streaming
 // hints apply to tiles whose rows are whole 128-byte lines (measured at 512^3
 // FP32: with 32- and 64-byte rows they double the pass's HBM traffic).
@@ -499,12 +513,13 @@ int launch_y(xpfc_ctx* c, void* spec, int v, int dir, const Batch& bt, int ny) {
 
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
   if (c->n1 > 1) LCHK(launch_y(c, spec, c->tc_y, -1, bt, ny));
-  if (do_x) LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, -1, c->tw0, bt, ny, false, c->stream));
+  if (do_x)
+    LCHK(launch_strided(c->prec, spec, x_segs(c, c->tc_x), c->n0, c->tc_x, -1, c->tw0, bt, ny, false, c->stream));
   return XPFC_OK;
 }
 
 int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
-  LCHK(launch_strided(c->prec, spec, c->segX, c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
+  LCHK(launch_strided(c->prec, spec, x_segs(c, c->tc_x), c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
   if (c->n1 > 1) LCHK(launch_y(c, spec, c->tc_y, +1, bt, ny));
   return XPFC_OK;
 }
@@ -749,7 +764,7 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
   const long long nl = (long long)c->n0 * c->n1;
   const Batch bt = bt_active(c);
   const int ny = (int)c->active.size();
-  const int nbx = launch_xstep(c->prec, c->work, c->psi, c->segX, c->n0, c->tc_x, c->tw0, c->g, bt, ny,
+  const int nbx = launch_xstep(c->prec, c->work, c->psi, x_segs(c, c->tc_x), c->n0, c->tc_x, c->tw0, c->g, bt, ny,
                                c->px, sample, c->stream);
   LCHK(nbx);
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
@@ -934,7 +949,7 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
   const Batch bt = bt_of(c, c->d_ids);
   auto launch = [&]() -> int {
     if (which == 0)
-      return launch_xstep(c->prec, a, b, c->segX, c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
+      return launch_xstep(c->prec, a, b, x_segs(c, v), c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
     if (which == 1) return launch_y(c, a, v, -1, bt, ny);
     return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, zl_own(c), v, bt, ny,
                         c->twz, c->twp, c->pz, false, c->stream);
