@@ -19,8 +19,10 @@ pytestmark = pytest.mark.gpu
 PRECS = [XPFC_FP64, XPFC_FP32_MIXED]
 TOL_FFT = {XPFC_FP64: 1e-12, XPFC_FP32_MIXED: 2e-6}
 
-SHAPES = [(16, 32), (64, 64), (256, 256), (1024, 64), (8, 16, 32), (32, 32, 32), (64, 16, 128),
-          (128, 128, 128)]
+# the largest lines (1024 points) along each axis are included: (1024, 64) axis 0,
+# (16, 1024) the real-space axis in 2-D, (8, 1024, 16) axis 1, (16, 8, 1024) axis 2
+SHAPES = [(16, 32), (64, 64), (256, 256), (1024, 64), (16, 1024), (8, 16, 32), (32, 32, 32),
+          (64, 16, 128), (128, 128, 128), (8, 1024, 16), (16, 8, 1024)]
 
 
 def _flush(msg):
@@ -72,7 +74,8 @@ def _run_both(op, lp, nsteps, amp=0.01, seed=0):
 # (8, 128, 128) and (128, 128, 128) have planes of >= 64 KiB: padded plane stride
 @pytest.mark.parametrize("shape,nsteps", [((32, 32), 60), ((64, 64), 80), ((16, 16, 16), 40),
                                           ((32, 16, 64), 30), ((8, 128, 128), 20),
-                                          ((128, 128, 128), 10)])
+                                          ((128, 128, 128), 10), ((16, 1024), 30),
+                                          ((8, 1024, 16), 10), ((16, 8, 1024), 10)])
 @pytest.mark.parametrize("prec", PRECS)
 def test_steps_vs_oracle_growth(shape, nsteps, prec):
     _flush("steps %s n=%d prec=%d" % (shape, nsteps, prec))
