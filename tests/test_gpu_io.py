@@ -106,3 +106,24 @@ def test_checkpoint_rejects_other_grids(tmp_path):
     with Context(q) as c:
         with pytest.raises(Exception):
             c.load_checkpoint(ck)
+
+
+def test_checkpoint_rejects_other_precision_and_damaged_files(tmp_path):
+    _, p32 = make((32, 32), prec=XPFC_FP32_MIXED)
+    _, p64 = make((32, 32), prec=XPFC_FP64)
+    ck = tmp_path / "s.ckpt"
+    with Context(p32) as c:
+        c.initialize_noise(0.01, 1)
+        c.save_checkpoint(str(ck))
+    with Context(p64) as c:
+        with pytest.raises(Exception):
+            c.load_checkpoint(str(ck))
+    data = ck.read_bytes()
+    (tmp_path / "short.ckpt").write_bytes(data[: len(data) // 2])
+    (tmp_path / "magic.ckpt").write_bytes(b"NOTACKPT" + data[8:])
+    with Context(p32) as c:
+        for bad in ("short.ckpt", "magic.ckpt", "missing.ckpt"):
+            with pytest.raises(Exception):
+                c.load_checkpoint(str(tmp_path / bad))
+        c.load_checkpoint(str(ck))  # the intact file still loads
+        assert c.current_step == 0
