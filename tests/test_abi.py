@@ -207,3 +207,15 @@ def test_instance_bytes_counts_three_spectra():
     assert 3 * spectrum32 <= b32 < 3 * spectrum32 + (1 << 20)
     assert 3 * 2 * spectrum32 <= b64 < 3 * 2 * spectrum32 + (1 << 20)
     assert X.instance_bytes(X.Params(n=(48, 48), L=(6.0, 6.0))) == 0
+
+
+def test_read_vtk_rejects_short_or_garbled_data(tmp_path):
+    psi = 0.05 + 0.01 * np.random.default_rng(1).standard_normal((6, 4))
+    text = O.vtk_text(psi, 12, 8)
+    lines = text.splitlines()
+    (tmp_path / "short.vtk").write_text("\n".join(lines[:-3]) + "\n")        # 3 values missing
+    (tmp_path / "nan.vtk").write_text("\n".join(lines[:-1] + ["abc"]) + "\n")  # a non-number
+    (tmp_path / "dims.vtk").write_text(text.replace("DIMENSIONS 6 4 1", "DIMENSIONS 6 -4 1"))
+    for name in ("short.vtk", "nan.vtk", "dims.vtk", "missing.vtk"):
+        with pytest.raises(X.XPFCError):
+            X.api.read_vtk(str(tmp_path / name))
