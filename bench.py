@@ -120,7 +120,19 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
     dt = time.perf_counter() - t0
     return {"value": done / dt, "unit": "steps/s", "cores": workers, "kind": "port",
             "sample": "%d step(s) of the %d^3 FP64 oracle (scipy.fft, %d workers) on %s"
-                      % (done, n, workers, platform.processor() or platform.machine())}
+                      % (done, n, workers, _cpu_model()),
+            "host_cpus": os.cpu_count()}
+
+
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 def config_runs():
