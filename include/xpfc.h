@@ -74,6 +74,12 @@ typedef struct xpfc_params {
   double h, k, roa;
   int device;                /* HIP device ordinal                              */
   int restart_order;         /* VTK restart mapping, see xpfc_inst_initialize_vtk */
+  int powf_fp32;             /* 1: psi^2, psi^3 (and psi^4 of the free energy) formed
+                                from float(psi) with FP32 products, as the reference's
+                                powf (power_functions.cu:47); 0: exact products in the
+                                working precision.  The reference's FP32 squaring of
+                                the residual (sse_calculation.cu:38) is not reproduced:
+                                the RMSE is a k-space FP64 sum. */
 } xpfc_params;
 
 /* One sampled diagnostic record (what sum_reduction.cu:64,77 prints). */
@@ -93,6 +99,9 @@ typedef struct xpfc_ctx xpfc_ctx;
 /* Defaults: the reference's hard-coded DCF (initialize.cu:10-17, sum mode),
  * a = 1, b = 1/3, Mo = 1, eps = 0, dt = 0.5, FP64. */
 int xpfc_params_default(xpfc_params* out);
+/* sizeof(xpfc_params) (no counterpart in the reference): a binding that
+ * mirrors the struct checks its layout against it. */
+size_t xpfc_params_size(void);
 /* read_input (read_input.cu:8-121): parse a pfc.input file, same positional
  * format; a = 1/a_in, b = 1/b_in.  Returns XPFC_ERR_IO / XPFC_ERR_PARSE. */
 int xpfc_read_input(const char* path, xpfc_params* out);
@@ -187,8 +196,8 @@ int xpfc_memcpy_d2h(xpfc_ctx* ctx, void* dst, const void* src, size_t bytes);
 size_t xpfc_real_bytes(const xpfc_ctx* ctx);      /* N reals                      */
 size_t xpfc_spectrum_bytes(const xpfc_ctx* ctx);  /* n0*n1*(n2/2+1) complex       */
 /* power_function_wrapper (power_functions.cu:4-53): dst = src^p on real fields
- * (exact products in the working precision; the reference's FP32 powf is an
- * oracle switch). */
+ * (exact products in the working precision, or FP32 products of float(src)
+ * when the context's powf_fp32 is set). */
 int xpfc_power_functions(xpfc_ctx* ctx, int p, const void* src, void* dst);
 /* convolution_wrapper (convolve_fields.cu:3-51): out = c2(k) * psi_k. */
 int xpfc_convolve_fields(xpfc_ctx* ctx, const void* psi_k, void* out);

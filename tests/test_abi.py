@@ -219,3 +219,14 @@ def test_read_vtk_rejects_short_or_garbled_data(tmp_path):
     for name in ("short.vtk", "nan.vtk", "dims.vtk", "missing.vtk"):
         with pytest.raises(X.XPFCError):
             X.api.read_vtk(str(tmp_path / name))
+
+
+def test_params_layout_and_powf_extension(tmp_path):
+    import ctypes
+    assert X.lib().xpfc_params_size() == ctypes.sizeof(X.api._Params)
+    f = tmp_path / "pfc.input"
+    f.write_text(PFC_INPUT + "powf_fp32 1\n")
+    p = X.read_input(str(f))
+    assert p.powf_fp32 == 1
+    assert X.Params.from_c(p.to_c()).powf_fp32 == 1
+    assert X.Params().powf_fp32 == 0

@@ -128,6 +128,37 @@ def test_non_default_physics_vs_oracle(shape, int_Mo, prec):
         assert abs(gfe[st] - r) <= (1e-11 if prec == XPFC_FP64 else 1e-7) * abs(r), (st, gfe[st], r)
 
 
+@pytest.mark.parametrize("shape", [(32, 32), (8, 16, 32)])
+@pytest.mark.parametrize("prec", PRECS)
+def test_fp32_powf_switch_vs_oracle(shape, prec):
+    """powf_fp32: the powers taken from float(psi) with FP32 products, as the
+    reference's powf (power_functions.cu:47), against the oracle's switch.  The
+    reference also squares the residual in FP32 (sse_calculation.cu:38); the
+    library's RMSE is a k-space FP64 sum, so RMSE is held to FP32 rounding."""
+    _flush("powf switch %s prec=%d" % (shape, prec))
+    op, lp = growth_case(shape, prec=prec, t_sample=10, powf_fp32=1)
+    psi0, ref, rs, got, gs = _run_both(op, lp, 30)
+    tol = 1e-10 if prec == XPFC_FP64 else 1e-5
+    assert np.max(np.abs(got - ref)) <= tol
+    rfe = {s.step: s.value for s in rs if s.kind == "FE"}
+    gfe = {s.step: s.F_ref for s in gs if s.has_fe}
+    assert sorted(gfe) == sorted(rfe) and rfe
+    for st, r in rfe.items():
+        assert abs(gfe[st] - r) <= (1e-11 if prec == XPFC_FP64 else 1e-7) * abs(r), (st, gfe[st], r)
+    rsse = {s.step: s.value for s in rs if s.kind == "SSE"}
+    gsse = {s.step: s.rmse for s in gs if s.has_sse}
+    for st, r in rsse.items():
+        assert abs(gsse[st] - r) <= 1e-5 * abs(r), (st, gsse[st], r)
+    if prec == XPFC_FP64:
+        # the switch is live: the clean FP64 run differs beyond FP64 rounding
+        _, lp0 = growth_case(shape, prec=prec, t_sample=10)
+        with Context(lp0) as c:
+            c.set_field(psi0)
+            c.step(30)
+            clean = c.get_field()
+        assert np.max(np.abs(clean - got)) > 1e-12
+
+
 @pytest.mark.parametrize("prec", PRECS)
 def test_c1_config_1000_steps(prec):
     """BASELINE 2-D square lattice, 256², FP64 plain (and FP32-mixed), 1000 steps."""

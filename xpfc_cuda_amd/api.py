@@ -53,7 +53,7 @@ class _Params(ct.Structure):
                 ("A_10", ct.c_double), ("A_11", ct.c_double), ("A_tri", ct.c_double),
                 ("iseed", ct.c_int), ("ishape", ct.c_int),
                 ("h", ct.c_double), ("k", ct.c_double), ("roa", ct.c_double),
-                ("device", ct.c_int), ("restart_order", ct.c_int)]
+                ("device", ct.c_int), ("restart_order", ct.c_int), ("powf_fp32", ct.c_int)]
 
 
 class _Sample(ct.Structure):
@@ -78,6 +78,7 @@ def load_library(path: Optional[str] = None):
     P = ct.POINTER
     sig = {
         "xpfc_params_default": (i, [P(_Params)]),
+        "xpfc_params_size": (sz, []),
         "xpfc_read_input": (i, [ct.c_char_p, P(_Params)]),
         "xpfc_reference_steps": (ll, [P(_Params)]),
         "xpfc_create": (i, [P(_Params), P(vp)]),
@@ -152,6 +153,9 @@ def load_library(path: Optional[str] = None):
         f = getattr(L, name)
         f.restype = res
         f.argtypes = args
+    if L.xpfc_params_size() != ct.sizeof(_Params):
+        raise ImportError("xpfc_params layout mismatch: library %d bytes, binding %d"
+                          % (L.xpfc_params_size(), ct.sizeof(_Params)))
     _lib = L
     return L
 
@@ -202,6 +206,7 @@ class Params:
     roa: float = 0.0
     device: int = 0
     restart_order: int = 0
+    powf_fp32: int = 0
 
     @property
     def dim(self) -> int:
@@ -218,7 +223,7 @@ class Params:
                   "h", "k", "roa"):
             setattr(c, f, float(getattr(self, f)))
         for f in ("dcf_mode", "precision", "t_sample", "int_Mo_residual", "no_val", "t_final",
-                  "iconfig", "iseed", "ishape", "device", "restart_order"):
+                  "iconfig", "iseed", "ishape", "device", "restart_order", "powf_fp32"):
             setattr(c, f, int(getattr(self, f)))
         if not 1 <= len(self.peaks) <= MAX_PEAKS:
             raise ValueError("1..4 DCF peaks")
@@ -238,7 +243,7 @@ class Params:
                    int_Mo_residual=c.int_Mo_residual, no_val=c.no_val, t_final=c.t_final,
                    iconfig=c.iconfig, A_10=c.A_10, A_11=c.A_11, A_tri=c.A_tri, iseed=c.iseed,
                    ishape=c.ishape, h=c.h, k=c.k, roa=c.roa, device=c.device,
-                   restart_order=c.restart_order)
+                   restart_order=c.restart_order, powf_fp32=c.powf_fp32)
 
 
 @dataclass

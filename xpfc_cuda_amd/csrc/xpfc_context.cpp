@@ -209,6 +209,7 @@ int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
   }
   in->ia.za.a = p.a;
   in->ia.za.b = p.b;
+  in->ia.za.powf32 = p.powf_fp32 ? 1 : 0;
   in->ia.za.scale = 1.0 / (double)c->N;
   in->ia.za.mean = in->mean;
   return XPFC_OK;
@@ -1348,7 +1349,7 @@ const Phys& ph0(const xpfc_ctx* c) { return c->inst[0].ia.ph; }
 extern "C" int xpfc_power_functions(xpfc_ctx* c, int p, const void* src, void* dst) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!src || !dst || p < 0) return fail(XPFC_ERR_ARG, "bad power_functions arguments");
-  LCHK(launch_power(c->prec, src, dst, c->N, p, c->stream));
+  LCHK(launch_power(c->prec, src, dst, c->N, p, c->inst[0].p.powf_fp32 != 0, c->stream));
   return op_done(c);
 }
 
@@ -1489,7 +1490,7 @@ extern "C" int xpfc_inst_save_checkpoint(xpfc_ctx* c, int s, const char* path) {
   const XInst& in = c->inst[s];
   CkptHeader h{};
   std::memcpy(h.magic, kMagic, 8);
-  h.version = 1;
+  h.version = 2;  // 2: xpfc_params with powf_fp32
   h.precision = c->prec;
   h.n0 = c->n0;
   h.n1 = c->n1;
@@ -1522,7 +1523,7 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
   const size_t bytes = (size_t)c->stride * c->csize;
   std::vector<char> buf(2 * bytes);
   bool ok = std::fread(&h, sizeof(h), 1, f) == 1;
-  ok = ok && std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 1;
+  ok = ok && std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 2;
   if (!ok) {
     std::fclose(f);
     return fail(XPFC_ERR_PARSE, "%s: not an XPFC checkpoint", path);

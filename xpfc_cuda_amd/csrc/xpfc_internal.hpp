@@ -60,6 +60,7 @@ struct ZArgs {
   double mean;   // FP64 mean density (the k = 0 mode)
   double a, b;   // N(psi) = -a psi^2 + b psi^3
   double scale;  // 1/N for the inverse transform
+  int powf32;    // powers from float(psi) with FP32 products (power_functions.cu:47)
 };
 
 enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
@@ -135,7 +136,7 @@ int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny
 int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s);
 
 // operator-level kernels on natural layouts (real [N], spectrum [n0][n1][n2/2+1])
-int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s);
+int launch_power(int prec, const void* src, void* dst, long long n, int p, bool f32, hipStream_t s);
 int launch_scale(int prec, void* buf, long long ncomplex_or_real, double f, hipStream_t s);
 int launch_convolve(int prec, const void* psi_k, void* out, const KGeom& g, const Phys& ph,
                     int n2, hipStream_t s);

@@ -11,13 +11,20 @@ namespace xpfc {
 // operator-level kernels on natural layouts
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void k_power(const T* __restrict__ src, T* __restrict__ dst, long long n, int p) {
+__global__ void k_power(const T* __restrict__ src, T* __restrict__ dst, long long n, int p, int f32) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const T x = src[i];
-    T r = (T)1;
-    for (int q = 0; q < p; ++q) r *= x;
-    dst[i] = r;
+    if (f32) {  // the reference's FP32 powf (power_functions.cu:47)
+      const float xf = (float)x;
+      float r = 1.0f;
+      for (int q = 0; q < p; ++q) r *= xf;
+      dst[i] = (T)r;
+    } else {
+      T r = (T)1;
+      for (int q = 0; q < p; ++q) r *= x;
+      dst[i] = r;
+    }
   }
 }
 
@@ -228,11 +235,13 @@ int launch_probe(int prec, const void* psi, long long stride, long long iny, int
   return launch_status();
 }
 
-int launch_power(int prec, const void* src, void* dst, long long n, int p, hipStream_t s) {
+int launch_power(int prec, const void* src, void* dst, long long n, int p, bool f32, hipStream_t s) {
   if (prec == 1)
-    hipLaunchKernelGGL(k_power<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, (float*)dst, n, p);
+    hipLaunchKernelGGL(k_power<float>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const float*)src, (float*)dst, n, p,
+                       f32 ? 1 : 0);
   else
-    hipLaunchKernelGGL(k_power<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, (double*)dst, n, p);
+    hipLaunchKernelGGL(k_power<double>, dim3(grid_for(n, 256)), dim3(256), 0, s, (const double*)src, (double*)dst, n,
+                       p, f32 ? 1 : 0);
   return launch_status();
 }
 

@@ -108,6 +108,8 @@ using xpfc::fail;
 // ---------------------------------------------------------------------------
 // parameters
 // ---------------------------------------------------------------------------
+extern "C" size_t xpfc_params_size(void) { return sizeof(xpfc_params); }
+
 extern "C" int xpfc_params_default(xpfc_params* out) {
   if (!out) return fail(XPFC_ERR_ARG, "null params");
   std::memset(out, 0, sizeof(*out));
@@ -190,6 +192,7 @@ extern "C" int xpfc_read_input(const char* path, xpfc_params* out) {
     else if (!std::strcmp(key, "Lz")) std::sscanf(rest, "%lf", &p.L[2]);
     else if (!std::strcmp(key, "Mz")) std::sscanf(rest, "%d", &p.n[2]);
     else if (!std::strcmp(key, "restart_order")) std::sscanf(rest, "%d", &p.restart_order);
+    else if (!std::strcmp(key, "powf_fp32")) std::sscanf(rest, "%d", &p.powf_fp32);
     else if (!std::strcmp(key, "precision")) {
       char v[32] = {0};
       std::sscanf(rest, "%31s", v);

@@ -257,6 +257,36 @@ template <int H, int E> struct ZIdx {
   }
 };
 
+// Free-energy sums (psi, psi^2, psi^3, psi^4) of two real-space values, and
+// the nonlinear term N(psi) = -a psi^2 + b psi^3.  With powf32 the powers are
+// those of the reference's FP32 powf (power_functions.cu:47): FP32 products of
+// float(psi); otherwise exact products (FP64 sums, working-precision N).
+__device__ __forceinline__ void fe_sums(double (&acc)[ZPART], double p0, double p1, int powf32) {
+  acc[0] += p0 + p1;
+  if (powf32) {
+    const float f0 = (float)p0, f1 = (float)p1;
+    const float s0 = f0 * f0, s1 = f1 * f1;
+    const float c0 = s0 * f0, c1 = s1 * f1;
+    acc[1] += (double)s0 + (double)s1;
+    acc[2] += (double)c0 + (double)c1;
+    acc[3] += (double)(c0 * f0) + (double)(c1 * f1);
+  } else {
+    acc[1] += p0 * p0 + p1 * p1;
+    acc[2] += p0 * p0 * p0 + p1 * p1 * p1;
+    acc[3] += (p0 * p0) * (p0 * p0) + (p1 * p1) * (p1 * p1);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ T nonlin(double p, T a, T b, const ZArgs& za) {
+  if (za.powf32) {
+    const float f = (float)p, sq = f * f, cu = sq * f;
+    return (T)(za.b * (double)cu - za.a * (double)sq);
+  }
+  const T q = (T)p;
+  return q * q * (b * q - a);
+}
+
 // The C2R pre-twiddle and the R2C post-twiddle pair bin k with bin H-k; the
 // mirror is exchanged with __shfl among the TPL lanes that hold the line (a
 // power of two <= 64, so a line never straddles two wavefronts).
@@ -334,12 +364,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
         const int k = t + m * TPL;
         const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
         if constexpr (SAMPLE) {
-          if (act) {
-            acc[0] += p0 + p1;
-            acc[1] += p0 * p0 + p1 * p1;
-            acc[2] += p0 * p0 * p0 + p1 * p1 * p1;
-            acc[3] += (p0 * p0) * (p0 * p0) + (p1 * p1) * (p1 * p1);
-          }
+          if (act) fe_sums(acc, p0, p1, za.powf32);
         }
         if (act) rp[k] = mk<C>((T)p0, (T)p1);
       }
@@ -351,15 +376,9 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
       for (int m = 0; m < E; ++m) {
         const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
         if constexpr (SAMPLE) {
-          if (act) {
-            acc[0] += p0 + p1;
-            acc[1] += p0 * p0 + p1 * p1;
-            acc[2] += p0 * p0 * p0 + p1 * p1 * p1;
-            acc[3] += (p0 * p0) * (p0 * p0) + (p1 * p1) * (p1 * p1);
-          }
+          if (act) fe_sums(acc, p0, p1, za.powf32);
         }
-        const T q0 = (T)p0, q1 = (T)p1;
-        v[m] = mk<C>(q0 * q0 * (b * q0 - a), q1 * q1 * (b * q1 - a));
+        v[m] = mk<C>(nonlin<T>(p0, a, b, za), nonlin<T>(p1, a, b, za));
       }
     }
   } else {
@@ -370,10 +389,8 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
     for (int m = 0; m < E; ++m) {
       const int k = t + m * TPL;
       C x = act ? rp[k] : mk<C>(0, 0);
-      if constexpr (MODE == Z_R2C_NL) {
-        const T q0 = (T)(za.mean + (double)x.x), q1 = (T)(za.mean + (double)x.y);
-        x = mk<C>(q0 * q0 * (b * q0 - a), q1 * q1 * (b * q1 - a));
-      }
+      if constexpr (MODE == Z_R2C_NL)
+        x = mk<C>(nonlin<T>(za.mean + (double)x.x, a, b, za), nonlin<T>(za.mean + (double)x.y, a, b, za));
       v[m] = act ? x : mk<C>(0, 0);
     }
   }
