@@ -212,6 +212,14 @@ int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
   in->ia.za.powf32 = p.powf_fp32 ? 1 : 0;
   in->ia.za.scale = 1.0 / (double)c->N;
   in->ia.za.mean = in->mean;
+  // the real-space kernels carry the FP32-power form only while an instance
+  // asks for it; captured step blocks bake the kernel choice in
+  bool any = false;
+  for (const XInst& x : c->inst) any = any || x.p.powf_fp32 != 0;
+  if (any != c->any_powf) {
+    c->any_powf = any;
+    drop_graphs(c);
+  }
   return XPFC_OK;
 }
 
@@ -536,7 +544,7 @@ int psi_to_real(xpfc_ctx* c, int s, bool sums) {
   if (rc) return rc;
   LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), c->aux, nullptr,
                     (long long)c->n0 * c->n1, c->H, zl_own(c), 256, bt, 1, c->twz, c->twp, c->pz,
-                    sums, c->stream));
+                    sums, c->any_powf, c->stream));
   return XPFC_OK;
 }
 
@@ -596,12 +604,12 @@ int field_from_aux(xpfc_ctx* c, int s) {
   void* work = inst_ptr(c, c->work, s);
   // psi_k = FFT(psi - mean): the k = 0 slot is then zero by construction
   LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, aux, nullptr, psi, ny_of(c, psi), nl, c->H, zl_own(c), 256,
-                    here, 1, c->twz, c->twp, nullptr, false, c->stream));
+                    here, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
   if ((rc = forward_xy(c, c->psi, bt, 1, true))) return rc;
   HIPCHK(hipMemsetAsync(psi, 0, c->csize, c->stream));
   // work = FFT_yz(N(psi)): the nonlinear term, all axes but axis 0
   LCHK(launch_zpass(c->prec, Z_R2C_NL, aux, nullptr, work, ny_of(c, work), nl, c->H, zl_own(c), 256,
-                    here, 1, c->twz, c->twp, nullptr, false, c->stream));
+                    here, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
   if ((rc = forward_xy(c, c->work, bt, 1, false))) return rc;
   HIPCHK(hipStreamSynchronize(c->stream));
   c->inst[s].have_field = true;
@@ -772,7 +780,7 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
   if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, +1, bt, ny));
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
   LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl, c->H,
-                    zl_own(c), c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->stream));
+                    zl_own(c), c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->any_powf, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
   if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, -1, bt, ny));
   if (ev) HIPCHK(hipEventRecord(ev[3], c->stream));
@@ -953,7 +961,7 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
       return launch_xstep(c->prec, a, b, x_segs(c, v), c->n0, v, c->tw0, c->g, bt, ny, c->px, false, c->stream);
     if (which == 1) return launch_y(c, a, v, -1, bt, ny);
     return launch_zpass(c->prec, Z_STEP, a, ny_of(c, a), a, ny_of(c, a), nl, c->H, zl_own(c), v, bt, ny,
-                        c->twz, c->twp, c->pz, false, c->stream);
+                        c->twz, c->twp, c->pz, false, c->any_powf, c->stream);
   };
   int r = launch();  // warm-up; also tells whether the variant exists
   if (r == -3) return -3;
@@ -1408,7 +1416,7 @@ extern "C" int xpfc_fft_r2c(xpfc_ctx* c, const void* real_in, void* spec_out) {
   // the caller's real field is contiguous; the spectrum uses the context layout
   const ZLines zin{c->n1, (long long)c->n1 * c->H, c->xs};
   LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, real_in, nullptr, c->aux, ny_of(c, c->aux), nl, c->H, zin, 256,
-                    bs, 1, c->twz, c->twp, nullptr, false, c->stream));
+                    bs, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
   if ((rc = forward_xy(c, c->aux, bt_one(c, 0), 1, true))) return rc;
   LCHK(launch_unpack(c->prec, c->aux, spec_out, (int)nl, c->H, c->n1, c->xs, c->stream));
   return op_done(c);
@@ -1425,7 +1433,7 @@ extern "C" int xpfc_fft_c2r(xpfc_ctx* c, const void* spec_in, void* real_out) {
   const Batch bs{c->d_ids + c->K, c->d_ia, 0, 0, 0};
   const ZLines zout{c->n1, c->xs, (long long)c->n1 * c->H};
   LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), real_out, nullptr, nl, c->H, zout, 256,
-                    bs, 1, c->twz, c->twp, nullptr, false, c->stream));
+                    bs, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
   return op_done(c);
 }
 

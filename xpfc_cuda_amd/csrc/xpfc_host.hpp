@@ -93,6 +93,7 @@ struct xpfc_ctx {
   xpfc::Segs segX{}, segY{};
   int tc_x = 0, tc_y = 0, ntz = 256;  // pass variants
   bool nt_y = false;  // axis-1 passes with streaming hints (fields larger than the last-level cache)
+  bool any_powf = false;  // some instance forms the FP32 powers (powf_fp32)
   bool tuned = false;
   bool use_graphs = true;
   std::vector<std::pair<std::pair<int, int>, hipGraphExec_t>> graphs;  // (steps, active) -> graph

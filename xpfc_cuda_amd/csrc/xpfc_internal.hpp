@@ -124,7 +124,8 @@ int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int t
                  hipStream_t s);
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                  long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
-                 const void* tw, const void* twp, double* partial, bool sample, hipStream_t s);
+                 const void* tw, const void* twp, double* partial, bool sample, bool pw,
+                 hipStream_t s);
 int zpass_blocks(int prec, int H, int ntz, long long nlines);
 int strided_blocks(int prec, int L, int tc, const Segs& segs);
 // out[slot * NSUMS + k]: the six sums of every listed instance

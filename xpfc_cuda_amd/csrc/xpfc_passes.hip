@@ -25,6 +25,8 @@
 // only how lines are grouped, ordered and cached, never the arithmetic of a
 // line, so every variant gives bitwise-identical fields; the autotuner picks
 // the fastest per size.
+#include <type_traits>
+
 #include "xpfc_device.hpp"
 
 namespace xpfc {
@@ -258,12 +260,13 @@ template <int H, int E> struct ZIdx {
 };
 
 // Free-energy sums (psi, psi^2, psi^3, psi^4) of two real-space values, and
-// the nonlinear term N(psi) = -a psi^2 + b psi^3.  With powf32 the powers are
-// those of the reference's FP32 powf (power_functions.cu:47): FP32 products of
+// the nonlinear term N(psi) = -a psi^2 + b psi^3.  F32: the powers are those
+// of the reference's FP32 powf (power_functions.cu:47), FP32 products of
 // float(psi); otherwise exact products (FP64 sums, working-precision N).
-__device__ __forceinline__ void fe_sums(double (&acc)[ZPART], double p0, double p1, int powf32) {
+template <bool F32>
+__device__ __forceinline__ void fe_sums(double (&acc)[ZPART], double p0, double p1) {
   acc[0] += p0 + p1;
-  if (powf32) {
+  if constexpr (F32) {
     const float f0 = (float)p0, f1 = (float)p1;
     const float s0 = f0 * f0, s1 = f1 * f1;
     const float c0 = s0 * f0, c1 = s1 * f1;
@@ -277,14 +280,29 @@ __device__ __forceinline__ void fe_sums(double (&acc)[ZPART], double p0, double 
   }
 }
 
-template <typename T>
+template <bool F32, typename T>
 __device__ __forceinline__ T nonlin(double p, T a, T b, const ZArgs& za) {
-  if (za.powf32) {
+  if constexpr (F32) {
     const float f = (float)p, sq = f * f, cu = sq * f;
     return (T)(za.b * (double)cu - za.a * (double)sq);
+  } else {
+    const T q = (T)p;
+    return q * q * (b * q - a);
   }
-  const T q = (T)p;
-  return q * q * (b * q - a);
+}
+
+// Runs body(std::true_type) for instances with the FP32 powers, else
+// body(std::false_type); PW = false compiles the default form only (a kernel
+// with both forms holds more registers, so it is launched only for contexts
+// that have an instance with the switch set).
+template <bool PW, class F>
+__device__ __forceinline__ void with_powers(const ZArgs& za, F&& body) {
+  if constexpr (PW) {
+    if (za.powf32) body(std::true_type{});
+    else body(std::false_type{});
+  } else {
+    body(std::false_type{});
+  }
 }
 
 // The C2R pre-twiddle and the R2C post-twiddle pair bin k with bin H-k; the
@@ -292,7 +310,7 @@ __device__ __forceinline__ T nonlin(double p, T a, T b, const ZArgs& za) {
 // power of two <= 64, so a line never straddles two wavefronts).
 // FREE: no occupancy hint (the compiler may use more registers instead of
 // spilling); a register-allocation variant only, the arithmetic is identical.
-template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE>
+template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE, bool PW>
 __global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
 void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
              cpx<T>* __restrict__ out_ny, long long nlines, ZLines zl, Batch bt,
@@ -359,40 +377,48 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
     fft_line<T, H, E, +1>(v, t, lds, idx, twl);
     if constexpr (MODE == Z_C2R) {
       C* rp = reinterpret_cast<C*>(out) + off_out;  // real pairs
+      with_powers<PW>(za, [&](auto f32) {
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const int k = t + m * TPL;
-        const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
-        if constexpr (SAMPLE) {
-          if (act) fe_sums(acc, p0, p1, za.powf32);
+        for (int m = 0; m < E; ++m) {
+          const int k = t + m * TPL;
+          const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
+          if constexpr (SAMPLE) {
+            if (act) fe_sums<decltype(f32)::value>(acc, p0, p1);
+          }
+          if (act) rp[k] = mk<C>((T)p0, (T)p1);
         }
-        if (act) rp[k] = mk<C>((T)p0, (T)p1);
-      }
+      });
       if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)blockIdx.x * ZPART);
       return;
     } else {
       const T a = (T)za.a, b = (T)za.b;
+      with_powers<PW>(za, [&](auto f32) {
+        constexpr bool F = decltype(f32)::value;
 #pragma unroll
-      for (int m = 0; m < E; ++m) {
-        const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
-        if constexpr (SAMPLE) {
-          if (act) fe_sums(acc, p0, p1, za.powf32);
+        for (int m = 0; m < E; ++m) {
+          const double p0 = za.mean + (double)v[m].x, p1 = za.mean + (double)v[m].y;
+          if constexpr (SAMPLE) {
+            if (act) fe_sums<F>(acc, p0, p1);
+          }
+          v[m] = mk<C>(nonlin<F, T>(p0, a, b, za), nonlin<F, T>(p1, a, b, za));
         }
-        v[m] = mk<C>(nonlin<T>(p0, a, b, za), nonlin<T>(p1, a, b, za));
-      }
+      });
     }
   } else {
     // input: the fluctuation psi - mean as real pairs (x[2k], x[2k+1])
     const C* rp = reinterpret_cast<const C*>(in) + off_in;
     const T a = (T)za.a, b = (T)za.b;
+    with_powers<PW>(za, [&](auto f32) {
+      constexpr bool F = decltype(f32)::value;
 #pragma unroll
-    for (int m = 0; m < E; ++m) {
-      const int k = t + m * TPL;
-      C x = act ? rp[k] : mk<C>(0, 0);
-      if constexpr (MODE == Z_R2C_NL)
-        x = mk<C>(nonlin<T>(za.mean + (double)x.x, a, b, za), nonlin<T>(za.mean + (double)x.y, a, b, za));
-      v[m] = act ? x : mk<C>(0, 0);
-    }
+      for (int m = 0; m < E; ++m) {
+        const int k = t + m * TPL;
+        C x = act ? rp[k] : mk<C>(0, 0);
+        if constexpr (MODE == Z_R2C_NL)
+          x = mk<C>(nonlin<F, T>(za.mean + (double)x.x, a, b, za), nonlin<F, T>(za.mean + (double)x.y, a, b, za));
+        v[m] = act ? x : mk<C>(0, 0);
+      }
+    });
   }
 
   // forward H-point transform and the real-to-half-spectrum post twiddle,
@@ -657,34 +683,39 @@ int zpass_blocks(int prec, int H, int ntz, long long nlines) {
   return prec == 1 ? zblocks_T<float>(H, ntz, nlines) : zblocks_T<double>(H, ntz, nlines);
 }
 
-template <typename T, int H, int NZ, bool FRE, int MODE>
+template <typename T, int H, int NZ, bool FRE, int MODE, bool PW = false>
 static void zlaunch(const void* in, const void* in_ny, void* out, void* out_ny, long long nlines,
                     const ZLines& zl, const Batch& bt, int ny, const void* tw, const void* twp,
                     double* partial, bool sample, hipStream_t s) {
   constexpr int G = NZ / z_tpl<T, H>();
   const dim3 grid((unsigned)((nlines + G - 1) / G), ny);
   if (sample)
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, true>), grid, dim3(NZ), 0, s, in,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, true, PW>), grid, dim3(NZ), 0, s, in,
                        (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, zl, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
   else
-    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, false>), grid, dim3(NZ), 0, s, in,
+    hipLaunchKernelGGL((k_zpass<T, H, NZ, FRE, MODE, false, PW>), grid, dim3(NZ), 0, s, in,
                        (const cpx<T>*)in_ny, out, (cpx<T>*)out_ny, nlines, zl, bt,
                        (const cpx<T>*)tw, (const cpx<T>*)twp, partial);
 }
 
-// ntz: workgroup size (64..512); +ZFREE_REGS drops the occupancy hint
+// ntz: workgroup size (64..512); +ZFREE_REGS drops the occupancy hint.
+// pw: some instance forms the FP32 powers (then the kernels that carry both
+// forms run, without the register-allocation variant)
 template <typename T>
 static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                    long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
-                   const void* tw, const void* twp, double* partial, bool sample, hipStream_t s) {
+                   const void* tw, const void* twp, double* partial, bool sample, bool pw,
+                   hipStream_t s) {
   if (ny == 0) return 0;
   const bool fre = (ntz & ZFREE_REGS) != 0;
   ntz &= ~ZFREE_REGS;
   if (mode == Z_STEP) {
     XPFC_LSWITCH(H, XPFC_NTZSWITCH(ntz, {
       if constexpr (z_ok<T, LL, NZ>()) {
-        if (fre)
+        if (pw)
+          zlaunch<T, LL, NZ, false, Z_STEP, true>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
+        else if (fre)
           zlaunch<T, LL, NZ, true, Z_STEP>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
         else
           zlaunch<T, LL, NZ, false, Z_STEP>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
@@ -699,8 +730,14 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
       if constexpr (z_ok<T, LL, NZ>()) {
         switch (mode) {
           case Z_R2C_FLUCT: zlaunch<T, LL, NZ, false, Z_R2C_FLUCT>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
-          case Z_R2C_NL: zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s); break;
-          case Z_C2R: zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s); break;
+          case Z_R2C_NL:
+            if (pw) zlaunch<T, LL, NZ, false, Z_R2C_NL, true>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s);
+            else zlaunch<T, LL, NZ, false, Z_R2C_NL>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, false, s);
+            break;
+          case Z_C2R:
+            if (pw) zlaunch<T, LL, NZ, false, Z_C2R, true>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
+            else zlaunch<T, LL, NZ, false, Z_C2R>(in, in_ny, out, out_ny, nlines, zl, bt, ny, tw, twp, partial, sample, s);
+            break;
           default: return -1;
         }
       } else {
@@ -713,10 +750,10 @@ static int zpass_T(int mode, const void* in, const void* in_ny, void* out, void*
 
 int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* out, void* out_ny,
                  long long nlines, int H, const ZLines& zl, int ntz, const Batch& bt, int ny,
-                 const void* tw, const void* twp, double* partial, bool sample, hipStream_t s) {
+                 const void* tw, const void* twp, double* partial, bool sample, bool pw, hipStream_t s) {
   return prec == 1
-             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, s)
-             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, s);
+             ? zpass_T<float>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, pw, s)
+             : zpass_T<double>(mode, in, in_ny, out, out_ny, nlines, H, zl, ntz, bt, ny, tw, twp, partial, sample, pw, s);
 }
 
 int launch_finalize(const double* px, int nx, const double* pz, int nz, const Batch& bt, int ny,
