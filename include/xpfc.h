@@ -49,7 +49,7 @@ extern "C" {
 /* Parameters.  Field names follow read_input.cu:19-115 and globals.cuh:40-52. */
 typedef struct xpfc_params {
   int dim;                   /* 2 or 3 (the reference is 2-D, globals.cuh:26) */
-  int n[3];                  /* Mx, My, Mz (Mz = 1 in 2-D); powers of two, 4..1024 */
+  int n[3];                  /* Mx, My, Mz (Mz = 1 in 2-D); powers of two, 8..1024 */
   double L[3];               /* Lx, Ly, Lz box lengths (ints in the reference) */
   double dt;                 /* time step                                       */
   double Mo;                 /* mobility                                        */
@@ -103,7 +103,11 @@ int xpfc_params_default(xpfc_params* out);
  * mirrors the struct checks its layout against it. */
 size_t xpfc_params_size(void);
 /* read_input (read_input.cu:8-121): parse a pfc.input file, same positional
- * format; a = 1/a_in, b = 1/b_in.  Returns XPFC_ERR_IO / XPFC_ERR_PARSE. */
+ * format; a = 1/a_in, b = 1/b_in.  Optional "key value" lines after the
+ * reference's lines (none in the reference's format): dim, Lz, Mz, precision
+ * (fp64 | fp32), dcf (sum | max), peaks (n, then k alpha rho_beta per peak),
+ * int_Mo_residual, powf_fp32, restart_order.  Returns XPFC_ERR_IO /
+ * XPFC_ERR_PARSE. */
 int xpfc_read_input(const char* path, xpfc_params* out);
 /* N_t_steps = t/dt + 1 truncated as the loop bound uses it (initialize.cu:60). */
 long long xpfc_reference_steps(const xpfc_params* p);

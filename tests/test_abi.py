@@ -225,8 +225,8 @@ def test_params_layout_and_powf_extension(tmp_path):
     import ctypes
     assert X.lib().xpfc_params_size() == ctypes.sizeof(X.api._Params)
     f = tmp_path / "pfc.input"
-    f.write_text(PFC_INPUT + "powf_fp32 1\n")
+    f.write_text(PFC_INPUT + "powf_fp32 1\nint_Mo_residual 0\ndcf max\n")
     p = X.read_input(str(f))
-    assert p.powf_fp32 == 1
+    assert p.powf_fp32 == 1 and p.int_Mo_residual == 0 and p.dcf_mode == X.XPFC_DCF_MAX
     assert X.Params.from_c(p.to_c()).powf_fp32 == 1
     assert X.Params().powf_fp32 == 0

@@ -193,6 +193,7 @@ extern "C" int xpfc_read_input(const char* path, xpfc_params* out) {
     else if (!std::strcmp(key, "Mz")) std::sscanf(rest, "%d", &p.n[2]);
     else if (!std::strcmp(key, "restart_order")) std::sscanf(rest, "%d", &p.restart_order);
     else if (!std::strcmp(key, "powf_fp32")) std::sscanf(rest, "%d", &p.powf_fp32);
+    else if (!std::strcmp(key, "int_Mo_residual")) std::sscanf(rest, "%d", &p.int_Mo_residual);
     else if (!std::strcmp(key, "precision")) {
       char v[32] = {0};
       std::sscanf(rest, "%31s", v);
