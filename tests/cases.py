@@ -8,7 +8,7 @@ TWO_PI = 2.0 * math.pi
 
 
 def square_peaks(rho_beta=1.0):
-    """BASELINE square lattice: k1 = 2π, k2 = 2√2π, α = 1, exp(-σ²k²/(2ρβ))."""
+    """Square lattice: k1 = 2π, k2 = 2√2π, α = 1, exp(-σ²k²/(2ρβ))."""
     return [(TWO_PI, 1.0, rho_beta), (TWO_PI * math.sqrt(2.0), 1.0, rho_beta)]
 
 

@@ -161,7 +161,7 @@ def test_fp32_powf_switch_vs_oracle(shape, prec):
 
 @pytest.mark.parametrize("prec", PRECS)
 def test_c1_config_1000_steps(prec):
-    """BASELINE 2-D square lattice, 256², FP64 plain (and FP32-mixed), 1000 steps."""
+    """The 2-D square-lattice workload: 256², FP64 plain (and FP32-mixed), 1000 steps."""
     _flush("c1 prec=%d" % prec)
     op, lp = make((256, 256), sigma=0.1, prec=prec, t_sample=100)
     psi0 = O.noise_ic(op, 0.01, 0)
