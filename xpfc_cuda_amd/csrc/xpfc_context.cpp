@@ -975,11 +975,30 @@ int time_pass(xpfc_ctx* c, int which, int v, void* a, void* b, int ny, int reps,
   } ev;
   HIPCHK(hipEventCreate(&ev.e0));
   HIPCHK(hipEventCreate(&ev.e1));
-  // average over n launches
+  // average over n launches replayed from a captured graph, as the step runs
+  // them: launched one by one from the host, kernels of a few microseconds
+  // (small grids) are all timed at the launch rate and cannot be told apart
   auto timed = [&](int n, float* out) -> int {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int lr = 0;
+    for (int i = 0; i < n && lr >= 0; ++i) lr = launch();
+    const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+    if (lr < 0 || ec != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return fail(XPFC_ERR_DEVICE, "autotune capture failed");
+    }
+    hipGraphExec_t ex = nullptr;
+    const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "autotune graph: %s", hipGetErrorString(ei));
+    struct Exec {
+      hipGraphExec_t e;
+      ~Exec() { (void)hipGraphExecDestroy(e); }
+    } hold{ex};
+    HIPCHK(hipGraphLaunch(ex, c->stream));  // warm-up replay
     HIPCHK(hipEventRecord(ev.e0, c->stream));
-    for (int i = 0; i < n; ++i)
-      if (launch() < 0) return fail(XPFC_ERR_DEVICE, "autotune launch failed");
+    HIPCHK(hipGraphLaunch(ex, c->stream));
     HIPCHK(hipEventRecord(ev.e1, c->stream));
     HIPCHK(hipEventSynchronize(ev.e1));
     HIPCHK(hipEventElapsedTime(out, ev.e0, ev.e1));
