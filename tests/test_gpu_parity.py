@@ -101,6 +101,24 @@ def test_steps_vs_oracle_growth(shape, nsteps, prec):
         assert abs(gsse[st] - r) <= (1e-9 if prec == XPFC_FP64 else 1e-4) * abs(r) + 1e-300, (st, gsse[st], r)
 
 
+@pytest.mark.parametrize("shape", [(8, 8), (16, 16), (8, 8, 8), (16, 8, 16)])
+@pytest.mark.parametrize("nsteps", [1, 10])
+@pytest.mark.parametrize("prec", PRECS)
+def test_smallest_grids_vs_oracle(shape, nsteps, prec):
+    """The smallest grids (8-point lines, 4-point half lines) after 1 and 10
+    steps, with a sample on every step."""
+    _flush("small %s n=%d prec=%d" % (shape, nsteps, prec))
+    op, lp = growth_case(shape, prec=prec, t_sample=1)
+    psi0, ref, rs, got, gs = _run_both(op, lp, nsteps, amp=0.02, seed=4)
+    tol = 1e-12 if prec == XPFC_FP64 else 1e-6
+    assert np.max(np.abs(got - ref)) <= tol
+    rfe = {s.step: s.value for s in rs if s.kind == "FE"}
+    gfe = {s.step: s.F_ref for s in gs if s.has_fe}
+    assert sorted(gfe) == sorted(rfe) and rfe
+    for st, r in rfe.items():
+        assert abs(gfe[st] - r) <= (1e-11 if prec == XPFC_FP64 else 1e-7) * abs(r), (st, gfe[st], r)
+
+
 @pytest.mark.parametrize("shape", [(32, 64), (16, 16, 32)])
 @pytest.mark.parametrize("int_Mo", [1, 0])
 @pytest.mark.parametrize("prec", PRECS)
