@@ -170,20 +170,15 @@ def config_runs():
 def batched_run(steps=2000, ninst=64, seed=0):
     """The 64-instance batch: one multi-instance context per grid side, all
     classes queued at once on their own streams; wall time of the whole job."""
-    import numpy as np
     import xpfc_cuda_amd as X
     from xpfc_cuda_amd import serving as S
-    rng = np.random.default_rng(seed)
-    sides = rng.choice([128, 256, 512, 1024], size=ninst)
-    sig = rng.uniform(0.05, 0.2, ninst)
-    n0 = rng.uniform(0.02, 0.08, ninst)
+    draws = S.batch_jobs(ninst, steps, seed)
     groups = {}
-    for i in range(ninst):
-        groups.setdefault(int(sides[i]), []).append(i)
+    for j in draws:
+        groups.setdefault(j.side, []).append(j.jid)
     ctxs, cells = [], 0
     for side, ids in sorted(groups.items()):
-        jobs = [S.Job(jid=i, dim=2, side=side, sigma=float(sig[i]), n0=float(n0[i]), steps=steps,
-                      seed=i) for i in ids]
+        jobs = [draws[i] for i in ids]
         ps = []
         for j in jobs:
             p = S.job_params(j)

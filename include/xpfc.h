@@ -140,8 +140,11 @@ long long xpfc_num_cells(const xpfc_ctx* ctx);
  * reference schedule (SSE when n % t_sample == 0, FE when (n+1) % t_sample
  * == 0) are appended to the sample log.  The steps are queued on the context
  * stream and the call returns; the sampled sums are read back by xpfc_sync and
- * by the calls that read or change the sample log or an instance's state, and
- * a non-finite sum is reported there (XPFC_ERR_NONFINITE). */
+ * by the calls that read or change the sample log or an instance's state.  A
+ * non-finite sum is kept in the record (NaN / Inf) and never stops the run of
+ * any instance; xpfc_sync (and xpfc_time_steps) return XPFC_ERR_NONFINITE once
+ * for each instance whose sampled sums became non-finite since the last such
+ * report. */
 int xpfc_step(xpfc_ctx* ctx, int nsteps);
 long long xpfc_current_step(const xpfc_ctx* ctx);
 /* Sample log: number of records, copy up to max of them (both return minus a
@@ -180,7 +183,9 @@ int xpfc_query(xpfc_ctx* ctx);
  * (two spectral modes are tested; a non-finite value anywhere in the field
  * reaches every mode within one step).  finite[ninst]. */
 int xpfc_probe_finite(xpfc_ctx* ctx, int* finite);
-/* Block until all work of the context has finished; read back pending samples. */
+/* Block until all work of the context has finished; read back pending
+ * samples.  Returns XPFC_ERR_NONFINITE (after completing the read-back) when
+ * an instance's sampled sums became non-finite since the last report. */
 int xpfc_sync(xpfc_ctx* ctx);
 
 /* ---- diagnostics ----------------------------------------------------------- */

@@ -434,18 +434,43 @@ def philox4x32_10(ctr: np.ndarray, key: Tuple[int, int]) -> np.ndarray:
     return c
 
 
-def philox_normals(ncells: int, seed: int) -> np.ndarray:
-    """One N(0,1) per cell c: Philox4x32-10 with counter (c_lo, c_hi, 0, 0) and
-    key (seed_lo, seed_hi); u_i = (r_i + 0.5) 2^-32; Box-Muller
-    z = sqrt(-2 ln u0) cos(2π u1), in FP64."""
-    c = np.arange(ncells, dtype=np.uint64)
-    ctr = np.zeros((ncells, 4), dtype=np.uint64)
-    ctr[:, 0] = c & _U32
-    ctr[:, 1] = c >> np.uint64(32)
-    r = philox4x32_10(ctr, (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF))
-    u0 = (r[:, 0].astype(np.float64) + 0.5) * (2.0 ** -32)
-    u1 = (r[:, 1].astype(np.float64) + 0.5) * (2.0 ** -32)
-    return np.sqrt(-2.0 * np.log(u0)) * np.cos(2.0 * PI * u1)
+def _philox_words(c0, c1, key):
+    """philox4x32_10 on counters (c0, c1, 0, 0), one 32-bit word per array
+    (the same rounds as philox4x32_10, without the (n, 4) stacking); returns
+    the first two output words."""
+    k0 = np.uint64(key[0] & 0xFFFFFFFF)
+    k1 = np.uint64(key[1] & 0xFFFFFFFF)
+    m0 = np.uint64(PHILOX_M0)
+    m1 = np.uint64(PHILOX_M1)
+    s32 = np.uint64(32)
+    x0, x1 = c0, c1
+    x2 = np.zeros_like(c0)
+    x3 = np.zeros_like(c0)
+    for r in range(10):
+        if r > 0:
+            k0 = (k0 + np.uint64(PHILOX_W0)) & _U32
+            k1 = (k1 + np.uint64(PHILOX_W1)) & _U32
+        p0 = m0 * x0
+        p1 = m1 * x2
+        x0, x1, x2, x3 = ((p1 >> s32) ^ x1 ^ k0) & _U32, p1 & _U32, ((p0 >> s32) ^ x3 ^ k1) & _U32, p0 & _U32
+    return x0, x1
+
+
+def philox_normals(ncells: int, seed: int, first: int = 0) -> np.ndarray:
+    """One N(0,1) per cell c (c = first .. first+ncells-1): Philox4x32-10 with
+    counter (c_lo, c_hi, 0, 0) and key (seed_lo, seed_hi); u_i = (r_i + 0.5)
+    2^-32; Box-Muller z = sqrt(-2 ln u0) cos(2π u1), in FP64.  Worked in
+    chunks of 2^18 cells."""
+    key = (seed & 0xFFFFFFFF, (seed >> 32) & 0xFFFFFFFF)
+    out = np.empty(ncells, dtype=np.float64)
+    chunk = 1 << 18
+    for a in range(0, ncells, chunk):
+        c = np.arange(first + a, first + min(ncells, a + chunk), dtype=np.uint64)
+        r0, r1 = _philox_words(c & _U32, c >> np.uint64(32), key)
+        u0 = (r0.astype(np.float64) + 0.5) * (2.0 ** -32)
+        u1 = (r1.astype(np.float64) + 0.5) * (2.0 ** -32)
+        out[a:a + c.size] = np.sqrt(-2.0 * np.log(u0)) * np.cos(2.0 * PI * u1)
+    return out
 
 
 def noise_ic(p: Params, amplitude: float, seed: int, fix_mean: bool = True) -> np.ndarray:

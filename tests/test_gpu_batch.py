@@ -166,3 +166,42 @@ def test_batch_larger_than_the_device_fails_cleanly():
         c.initialize_noise(0.01, 1)
         c.step(5)
         assert np.all(np.isfinite(c.get_field()))
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+def test_divergence_of_one_instance_does_not_stop_the_others(prec):
+    """Instance 0 (64^2, sigma 0.062, n0 0.0564, seed 4: the explicit nonlinear
+    term blows up within ~30 steps) beside a healthy liquid, both sampled every
+    step for 1100 steps, so the held sums are read back twice in the middle of
+    the run.  The healthy instance must take all 1100 steps and end exactly
+    as alone; the divergence is reported once by sync, and its records keep
+    the non-finite values."""
+    from xpfc_cuda_amd import XPFCError
+    from xpfc_cuda_amd.api import XPFC_ERR_NONFINITE
+    shape, nsteps = (64, 64), 1100
+    _flush("divergence in a batch prec=%d" % prec)
+    _, bad = make(shape, prec=prec, sigma=0.062, psi0=0.0564, t_sample=1)
+    _, good = make(shape, prec=prec, sigma=0.1, psi0=0.05, t_sample=1)
+    with BatchContext([bad, good]) as bc:
+        v = bc.autotune()
+        bc.initialize_noise(0, 0.01, 4)
+        bc.initialize_noise(1, 0.01, 5)
+        bc.step(nsteps)
+        with pytest.raises(XPFCError) as ei:
+            bc.sync()
+        assert ei.value.status == XPFC_ERR_NONFINITE
+        bc.sync()  # reported once
+        assert bc.current_step(0) == nsteps and bc.current_step(1) == nsteps
+        f_good = bc.get_field(1)
+        s_bad, s_good = bc.samples(0), bc.samples(1)
+    assert not all(np.isfinite(r.F_ref) for r in s_bad if r.has_fe)
+    assert all(np.isfinite(r.F_ref) for r in s_good if r.has_fe)
+    assert len(s_good) == len(s_bad)
+    with Context(good) as c:
+        c.set_variants(*v)
+        c.initialize_noise(0.01, 5)
+        c.step(nsteps)
+        c.sync()
+        alone, s_alone = c.get_field(), c.samples()
+    assert np.array_equal(f_good, alone)
+    _same_samples(s_good, s_alone)

@@ -65,3 +65,63 @@ def test_cli_resume_from_checkpoint(tmp_path):
     resumed = _run(tmp_path, "--no-vtk", "--steps", "50", "--resume", "c.ckpt")
     assert resumed[-1] == straight[-1]
     assert resumed[-1].startswith("iteration: 90 F = ")
+
+
+LATTICE = """1          no_val
+0.05 {s}    psi_0 s
+1.0 3.0 0.0 1.0  a b eps Mo
+
+8 8        Lx Ly
+64 64      Mx My
+
+30         t
+0.5        dt
+20         t_sample
+
+{iconfig}          iconfig
+0.02 0.01  A_10 A_11
+0.02       A_tri
+
+0          iseed
+0          ishape
+0 0 0      h k roa
+"""
+
+
+def _numbers(lines):
+    """(step, F, SSE or None) of every printed record."""
+    out = []
+    for l in lines:
+        t = l.split()
+        sse = float(t[t.index("SSE_gpu") + 2]) if "SSE_gpu" in t else None
+        out.append((int(t[1]), float(t[4]), sse))
+    return out
+
+
+@pytest.mark.parametrize("iconfig,sigma,L", [(1, 0.18, 8.0), (0, 0.19, 9.0)])
+def test_cli_lines_match_the_oracle(tmp_path, iconfig, sigma, L):
+    """xpfc_run on a pfc.input with a lattice initial condition (square, or
+    triangular with the box scaled by a_tri and truncated to int: 8 -> 9) and
+    the reference's hard-coded DCF (sum of the peaks 2pi and 2sqrt2 pi, rho*B =
+    4 and 4/sqrt2): every printed F and SSE_gpu value equals the FP64 oracle's
+    run of the same input to the printed digits (F: 10 decimals; SSE: 6
+    significant digits)."""
+    from oracle import xpfc_oracle as O
+    (tmp_path / "pfc.input").write_text(LATTICE.format(s=sigma, iconfig=iconfig))
+    lines = _run(tmp_path, "--no-vtk")
+    op = O.Params(n=(64, 64), L=(L, L), sigma=sigma, psi0=0.05, a=1.0, b=1.0 / 3.0, eps=0.0,
+                  M=1.0, dt=0.5, peaks=O.reference_peaks(), dcf_mode="sum", t_sample=20)
+    psi = O.lattice_ic(op, iconfig, A_10=0.02, A_11=0.01, A_tri=0.02)
+    _, rs = O.run(psi, op, 61, sample=True)
+    fe = {}
+    sse = {}
+    for s in rs:
+        (fe if s.kind == "FE" else sse)[s.step] = s.value
+    got = _numbers(lines)
+    # FE + SSE at 0, 20, 40, 60 (one line each), the closing FE at 61
+    assert [g[0] for g in got] == [0, 20, 40, 60, 61]
+    for step, F, e in got:
+        assert abs(F - fe[step]) <= 1e-10 + 1e-11 * abs(fe[step]), (step, F, fe[step])
+        if e is not None:
+            assert abs(e - sse[step]) <= 1e-5 * abs(sse[step]), (step, e, sse[step])
+    assert sum(g[2] is not None for g in got) == 4

@@ -5,7 +5,8 @@ import numpy as np
 import pytest
 
 from oracle import xpfc_oracle as O
-from xpfc_cuda_amd import BatchContext, Context, XPFC_FP32_MIXED, XPFC_FP64
+from xpfc_cuda_amd import BatchContext, Context, XPFCError, XPFC_FP32_MIXED, XPFC_FP64
+from xpfc_cuda_amd.api import XPFC_ERR_ARG, XPFC_ERR_NONFINITE
 
 from cases import growth_case, make
 
@@ -106,6 +107,13 @@ def test_checkpoint_rejects_other_grids(tmp_path):
     with Context(q) as c:
         with pytest.raises(Exception):
             c.load_checkpoint(ck)
+    # same extents, other box length (dx = 0.1 instead of 0.125): the
+    # wavenumbers and the C2 table would disagree with the checkpoint's physics
+    _, r = make((32, 32), dx=0.1)
+    with Context(r) as c:
+        with pytest.raises(XPFCError) as ei:
+            c.load_checkpoint(ck)
+        assert ei.value.status == XPFC_ERR_ARG
 
 
 def test_checkpoint_rejects_other_precision_and_damaged_files(tmp_path):

@@ -282,6 +282,176 @@ def test_square_ic_values():
     assert abs(psi.mean() - 0.05) < 1e-16
 
 
+# ---------------------------------------------------------------- lattice ICs
+# lattice_ic returns the field after the mean fix, which adds one constant to
+# every cell: the checks below use differences between cells, or cells whose
+# raw value is known exactly, so they hold whatever that constant is.
+S3 = math.sqrt(3.0)
+
+
+def _box(nx, ny, Lx, Ly, psi0=0.05):
+    return O.Params(n=(nx, ny), L=(Lx, Ly), psi0=psi0)
+
+
+def test_triangular_lattice_special_points_and_mirrors():
+    """Triangular one-mode field A(cos(qx)/2 + cos(qx/2)cos(sqrt3 qy/2)), q = 2pi/a_tri
+    = sqrt3 pi (initialize.cu:116-118).  On x_i = i/(2 sqrt3), y_j = j/6 the
+    phases are qx = i pi/2 and sqrt3 qy/2 = j pi/4, so (by hand): all three
+    waves in phase at the origin, 3A/2; -A/2 at (0, 4), (4, 0), (2, 0), (2, 2);
+    A/sqrt2 at (1, 0); and the field is even about the box centre in x and y."""
+    A = 0.3
+    p = _box(9, 9, 4.0 / S3, 4.0 / 3.0)
+    psi = O.lattice_ic(p, 0, A_tri=A)
+    d = psi - psi[0, 0] + 1.5 * A          # raw deviation, fixed at the origin's 3A/2
+    for (i, j) in [(0, 4), (4, 0), (2, 0), (2, 2), (6, 2)]:
+        assert d[i, j] == pytest.approx(-0.5 * A, abs=1e-14), (i, j)
+    assert d[8, 8] == pytest.approx(1.5 * A, abs=1e-14)
+    assert d[1, 0] == pytest.approx(A / math.sqrt(2.0), abs=1e-14)   # cos(pi/2)/2 + cos(pi/4)
+    assert d[0, 1] == pytest.approx(A * (0.5 + 1.0 / math.sqrt(2.0)), abs=1e-14)
+    assert np.allclose(psi, psi[::-1, :], atol=1e-14)   # mirror in x about Lx/2
+    assert np.allclose(psi, psi[:, ::-1], atol=1e-14)   # mirror in y about Ly/2
+    assert abs(psi.mean() - 0.05) < 1e-16
+
+
+def test_square_lattice_special_points():
+    """Square field A10((3sqrt2-2)cos(2pi y) - 2cos(2pi x)) - A11 cos(2pi x)cos(2pi y)
+    (initialize.cu:96-99) on x_i = i/4, y_j = j/4 (cosines 1, 0, -1, 0, 1):
+    by hand, psi(2,0)-psi(0,0) = 4A10 + 2A11, psi(0,2)-psi(0,0) = -2(3sqrt2-2)A10
+    + 2A11, psi(1,1)-psi(0,0) = -(3sqrt2-4)A10 + A11.  Swapping x and y, or the
+    sign of either amplitude, changes one of them."""
+    A10, A11 = 0.07, 0.03
+    psi = O.lattice_ic(_box(5, 5, 1.0, 1.0), 1, A_10=A10, A_11=A11)
+    r = 3.0 * math.sqrt(2.0)
+    assert psi[2, 0] - psi[0, 0] == pytest.approx(4 * A10 + 2 * A11, abs=1e-15)
+    assert psi[0, 2] - psi[0, 0] == pytest.approx(-2 * (r - 2) * A10 + 2 * A11, abs=1e-15)
+    assert psi[1, 1] - psi[0, 0] == pytest.approx(-(r - 4) * A10 + A11, abs=1e-15)
+    assert psi[4, 4] == pytest.approx(psi[0, 0], abs=1e-15)
+    assert abs(psi.mean() - 0.05) < 1e-16
+
+
+@pytest.mark.parametrize("iconfig,a_lat", [(0, 2.0 / S3), (1, 1.0)])
+def test_circle_seed(iconfig, a_lat):
+    """initialize.cu:158-172: cells farther than r = roa * a (a = a_tri or a_sq)
+    from (h, k) are psi0; nearer cells keep the lattice value.  After the mean
+    fix: every outside cell holds one value, and inside cells differ from the
+    unseeded field by one constant.  Cells with roa < d <= roa*a_tri must be
+    inside for the triangular lattice (the radius uses the lattice constant)."""
+    p = _box(33, 33, 4.0, 4.0)
+    roa, h, kc = 1.0, 2.0, 2.0
+    kw = dict(A_10=0.07, A_11=0.03, A_tri=0.2)
+    seeded = O.lattice_ic(p, iconfig, iseed=1, ishape=0, h=h, kc=kc, roa=roa, **kw)
+    plain = O.lattice_ic(p, iconfig, **kw)
+    x = np.arange(33) * (4.0 / 32)
+    d = np.sqrt((x[:, None] - h) ** 2 + (x[None, :] - kc) ** 2)
+    out, ins = d > roa * a_lat, d <= roa * a_lat
+    assert np.ptp(seeded[out]) < 1e-15
+    diff = seeded[ins] - plain[ins]
+    assert np.ptp(diff) < 1e-14
+    if iconfig == 1:
+        # the centre cell (x = y = 2, all cosines 1) has the raw value
+        # A10(3sqrt2 - 4) - A11 + psi0, which fixes the seeded field's shift:
+        # the outside cells then hold psi0 before it
+        raw_c = 0.07 * (3.0 * math.sqrt(2.0) - 4.0) - 0.03 + 0.05
+        shift = seeded[16, 16] - raw_c
+        assert seeded[out][0] - shift == pytest.approx(0.05, abs=1e-15)
+    ring = (d > roa) & (d <= roa * a_lat)
+    if iconfig == 0:
+        assert ring.sum() > 0
+        assert np.ptp(seeded[ring] - plain[ring]) < 1e-14   # lattice values kept there
+
+
+def test_gaussian_seed():
+    """initialize.cu:174-188: psi <- (psi - psi0) G + psi0 (1 - G), G =
+    exp(-|r - (h,k)|^2 / roa) (roa is the variance, not its square root).  On
+    the triangular grid of the special-points test, 8 periods wide, with the
+    centre on a lattice maximum: far cells (G = 0 exactly) hold psi0 before the
+    shift; the centre holds the lattice deviation 3A/2 (not 3A/2 + psi0); the
+    next cell along x holds (A/sqrt2) G + psi0 (1 - G) with G = exp(-1/(12 roa))."""
+    A, psi0, roa = 0.2, 0.05, 0.05
+    p = _box(65, 65, 32.0 / S3, 32.0 / 3.0, psi0=psi0)
+    x = np.arange(65) * (p.L[0] / 64)
+    y = np.arange(65) * (p.L[1] / 64)
+    h, kc = x[32], y[32]
+    psi = O.lattice_ic(p, 0, A_tri=A, iseed=1, ishape=1, h=h, kc=kc, roa=roa)
+    far = psi[0, 0]
+    shift = far - psi0
+    assert psi[64, 64] == far and psi[0, 64] == far
+    assert psi[32, 32] - shift == pytest.approx(1.5 * A, abs=1e-14)
+    G = math.exp(-1.0 / (12.0 * roa))
+    assert psi[33, 32] - shift == pytest.approx(A / math.sqrt(2.0) * G + psi0 * (1 - G), abs=1e-14)
+    assert psi[31, 32] == pytest.approx(psi[33, 32], abs=1e-14)
+
+
+# ---------------------------------------------------------------- powf_fp32 bound
+POWF_ULP = 4   # CUDA's documented maximum error of powf (full range), in ulp
+
+
+@pytest.mark.parametrize("pw", [2, 3, 4])
+def test_powf_fp32_within_cuda_documented_bound(pw):
+    """power_functions.cu:47 computes powf((float)psi, p).  The oracle's switch
+    forms float products of float(psi); each result must be a float and lie
+    within POWF_ULP ulp of the exact power of float(psi) (evaluated with exact
+    rational arithmetic), the bound CUDA documents for powf itself; so the
+    switch and CUDA's powf differ by at most twice that."""
+    from fractions import Fraction
+    rng = np.random.default_rng(pw)
+    x = np.concatenate([rng.uniform(-1.5, 1.5, 2000), rng.uniform(-0.1, 0.1, 2000),
+                        [0.0, 1.0, -1.0, 0.05, 1e-20]])
+    got = O.power_function(x, pw, powf_fp32=True)
+    assert np.array_equal(got, got.astype(np.float32).astype(np.float64))  # FP32 values
+    xf = x.astype(np.float32)
+    worst = 0.0
+    for g, v in zip(got, xf):
+        exact = Fraction(float(v)) ** pw
+        ex32 = np.float32(float(exact))
+        ulp = float(np.spacing(np.abs(ex32))) if ex32 != 0 else float(np.finfo(np.float32).tiny)
+        err = abs(Fraction(float(g)) - exact) / Fraction(ulp)
+        worst = max(worst, float(err))
+    assert worst <= POWF_ULP, worst
+    # the slip of not rounding the input to float first is visible
+    assert not np.array_equal(got, (x ** pw).astype(np.float32).astype(np.float64))
+
+
+# ---------------------------------------------------------------- divergence
+def _square(n, sigma, psi0):
+    return O.Params(n=n, L=tuple(m * 0.125 for m in n), sigma=sigma, psi0=psi0,
+                    peaks=[O.Peak(TWO_PI, 1.0, 1.0), O.Peak(TWO_PI * math.sqrt(2), 1.0, 1.0)],
+                    dcf_mode="max", t_sample=10 ** 6)
+
+
+def _blowup(p, amp, seed, nmax, record=()):
+    st = O.Stepper(O.noise_ic(p, amp, seed), p)
+    amp_at = {}
+    with np.errstate(over="ignore", invalid="ignore"):
+        for s in range(1, nmax + 1):
+            st.step()
+            a = float(np.max(np.abs(st.psi - p.psi0)))
+            if s in record:
+                amp_at[s] = a
+            if not np.isfinite(a):
+                return s, amp_at
+    return None, amp_at
+
+
+def test_oracle_diverges_on_the_strongly_crystallising_job():
+    """The service test's job 4 (64^2, sigma 0.062, n0 0.0564, noise 0.01 seed 4)
+    diverges under the reference's explicit nonlinear term in FP64 too: the
+    amplitude is 0.03 at step 10, 0.13 at 15, 1.3 at 19 and the field is
+    non-finite before step 40 (well inside the job's 137 steps)."""
+    s, a = _blowup(_square((64, 64), 0.062, 0.0564), 0.01, 4, 137, record=(10, 15, 19))
+    assert s is not None and s < 40
+    assert 0.02 < a[10] < 0.05 and 0.1 < a[15] < 0.2 and 1.0 < a[19] < 2.0
+
+
+def test_oracle_diverges_on_the_slow_growth_case():
+    """The sigma = 0.07 growth case (32^2, n0 0.05, seed 1) used by the GPU
+    parity tests for up to 80 steps: amplitude 0.002 at step 20 and 0.04 at
+    step 100, non-finite shortly after (between 100 and 160)."""
+    s, a = _blowup(_square((32, 32), 0.07, 0.05), 0.01, 1, 160, record=(20, 80, 100))
+    assert s is not None and 100 < s <= 160
+    assert a[20] < 0.003 and a[80] < 0.03 and 0.02 < a[100] < 0.1
+
+
 def test_vtk_layout_x_fastest():
     psi = np.arange(6, dtype=float).reshape(3, 2)  # nx = 3, ny = 2
     t = O.vtk_text(psi, 3, 4).splitlines()

@@ -172,3 +172,12 @@ def test_blocks_lengthen_only_while_no_job_waits(fake):
     assert {j.jid: j.F_ref for j in svc.done} == {0: 130, 1: 470}
     with pytest.raises(ValueError):
         S.Service(lookahead=0)
+
+
+def test_job_larger_than_the_budget_is_rejected_at_submission(fake):
+    """A job that could never be admitted must not block the queue forever."""
+    svc = S.Service(budget_bytes=128 * 128 - 1, slots=lambda d, s: 2)
+    with pytest.raises(ValueError):
+        svc.submit(_job(0, 128, 10))
+    svc.submit(_job(1, 64, 10))  # a job that fits is still accepted
+    assert svc.queued == 1

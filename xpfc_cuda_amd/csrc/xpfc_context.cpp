@@ -16,6 +16,7 @@
 #include <cstring>
 #include <fstream>
 #include <map>
+#include <new>
 #include <mutex>
 #include <string>
 #include <thread>
@@ -164,6 +165,26 @@ bool same_grid(const xpfc_params& a, const xpfc_params& b) {
   return true;
 }
 
+// A peak term below exp(-ecut) is dropped: 1e-10 (FP32 fields) / 4e-18
+// (FP64) of the denominator, which is >= 1.
+double peak_ecut(int prec) { return prec == 1 ? 23.0 : 40.0; }
+
+// |k|^2 beyond which every peak term of p is below exp(-ecut)
+double peak_kcut2(const xpfc_params& p, double ecut) {
+  double kc = 0.0;
+  for (int i = 0; i < p.npeaks && i < XPFC_MAX_PEAKS; ++i)
+    kc = std::max(kc, p.peak_k[i] + std::sqrt(ecut * 2.0 * p.peak_alpha[i] * p.peak_alpha[i]));
+  return kc * kc;
+}
+
+// Entries of the C2 table over m = |k|^2 / dk^2 of a square / cubic grid (up
+// to the cutoff, or the largest m on the grid); axis extents (n0, n1, n2).
+int c2_table_entries(double kcut2, double dksq, int n0, int n1, int n2) {
+  const double mmax = (double)(n0 / 2) * (n0 / 2) + (double)(n1 / 2) * (n1 / 2) +
+                      (double)(n2 / 2) * (n2 / 2);
+  return (int)std::min(std::ceil(kcut2 / dksq) + 1.0, mmax + 1.0);
+}
+
 // physics of one instance -> its kernel table entry
 int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
   if (p.npeaks < 1 || p.npeaks > XPFC_MAX_PEAKS) return fail(XPFC_ERR_ARG, "npeaks out of range");
@@ -177,23 +198,16 @@ int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
   ph.Mres = p.int_Mo_residual ? (double)(int)p.Mo : p.Mo;
   ph.npk = p.npeaks;
   ph.dcf_max = p.dcf_mode == XPFC_DCF_MAX;
-  // a peak term below exp(-ecut) is dropped: 1e-10 (FP32 fields) / 4e-18 (FP64)
-  // of the denominator, which is >= 1
-  ph.ecut = c->prec == 1 ? 23.0 : 40.0;
-  double kc = 0.0;
+  ph.ecut = peak_ecut(c->prec);
   for (int i = 0; i < p.npeaks; ++i) {
     ph.pk[i] = p.peak_k[i];
     ph.inv2a2[i] = 1.0 / (2.0 * p.peak_alpha[i] * p.peak_alpha[i]);
     ph.amp[i] = std::exp(-(p.sigma * p.peak_k[i]) * (p.sigma * p.peak_k[i]) / (2.0 * p.peak_rho_beta[i]));
-    kc = std::max(kc, p.peak_k[i] + std::sqrt(ph.ecut / ph.inv2a2[i]));
   }
-  ph.kcut2 = kc * kc;
+  ph.kcut2 = peak_kcut2(p, ph.ecut);
   in->c2zero = c2hat_host(0.0, p);
   if (c->g.cubic) {
-    // C2 table over m = |k|^2 / dk^2 up to the cutoff (or the largest m on the grid)
-    const double mmax = (double)(c->n0 / 2) * (c->n0 / 2) + (double)(c->n1 / 2) * (c->n1 / 2) +
-                        (double)c->H * c->H;
-    const int mcut = (int)std::min(std::ceil(ph.kcut2 / c->g.dksq) + 1.0, mmax + 1.0);
+    const int mcut = c2_table_entries(ph.kcut2, c->g.dksq, c->n0, c->n1, c->n2);
     if (mcut > in->c2cap) {
       HIPCHK(hipStreamSynchronize(c->stream));  // queued steps may still read the old table
       if (in->c2tab) (void)hipFree(in->c2tab);
@@ -246,8 +260,9 @@ void* inst_ptr(const xpfc_ctx* c, void* base, int s) {
 }
 
 // Reads the sums of pending sampled steps back into the sample logs (defined
-// with the step driver).
+// with the step driver); drain_report also reports newly diverged instances.
 int drain(xpfc_ctx* c);
+int drain_report(xpfc_ctx* c);
 // Builds instance s's spectra from the fluctuation held in its aux region
 // (defined with the field I/O).
 int field_from_aux(xpfc_ctx* c, int s);
@@ -363,6 +378,11 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
       hipMalloc((void**)&c->d_active, (size_t)K * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&c->d_flags, (size_t)K * sizeof(int)) != hipSuccess)
     return bail(fail(XPFC_ERR_NOMEM, "table allocation failed"));
+  try {
+    c->hsums.assign((size_t)SUM_ROWS * K * NSUMS, 0.0);  // read-back rows of drain
+  } catch (const std::bad_alloc&) {
+    return bail(fail(XPFC_ERR_NOMEM, "host allocation failed"));
+  }
 
   c->g.n0 = c->n0;
   c->g.n1 = c->n1;
@@ -418,8 +438,15 @@ extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
   int npx = 4096, npz = 1;
   for (int tc : {4, 8, 16}) npx = std::max(npx, strided_blocks(prec, n0, tc, sx));
   for (int nz : {64, 128, 256, 512}) npz = std::max(npz, zpass_blocks(prec, H, nz, (long long)n0 * n1));
+  // the instance's C2 table on square / cubic grids (as build_inst sizes it)
+  size_t c2bytes = 0;
+  const double L0 = p->L[0], L1 = p->dim == 3 ? p->L[1] : L0, L2 = p->dim == 3 ? p->L[2] : p->L[1];
+  const double dk0 = 2.0 * M_PI / L0, dk1 = 2.0 * M_PI / L1, dk2 = 2.0 * M_PI / L2;
+  if (dk0 == dk2 && dk1 == dk0 && p->npeaks >= 1 && p->npeaks <= XPFC_MAX_PEAKS)
+    c2bytes = (size_t)c2_table_entries(peak_kcut2(*p, peak_ecut(prec)), dk0 * dk0, n0, n1, n2) *
+              sizeof(double);
   return 3 * (size_t)(nyoff + nny) * csize + ((size_t)npx * XPART + (size_t)npz * ZPART) * sizeof(double) +
-         (size_t)(SUM_ROWS + 1) * NSUMS * sizeof(double) + 2 * sizeof(InstArgs) + 3 * sizeof(int);
+         (size_t)(SUM_ROWS + 1) * NSUMS * sizeof(double) + 2 * sizeof(InstArgs) + 3 * sizeof(int) + c2bytes;
 }
 
 extern "C" void xpfc_destroy(xpfc_ctx* c) {
@@ -466,6 +493,7 @@ extern "C" int xpfc_inst_reset(xpfc_ctx* c, int s) {
   in.samples.clear();
   in.have_field = false;
   in.have_rmse = false;
+  in.nonfinite = in.nonfinite_reported = false;
   return XPFC_OK;
 }
 
@@ -901,24 +929,48 @@ void add_sse_record(xpfc_ctx* c, XInst& in, long long step, double sse) {
   in.samples.push_back(r);
 }
 
+// Non-finite sums are recorded as they are (NaN / Inf in the records) and
+// flagged per instance; the run of the other instances goes on.  xpfc_sync
+// reports each newly diverged instance once (XPFC_ERR_NONFINITE).
 int drain(xpfc_ctx* c) {
   if (c->pending.empty()) return XPFC_OK;
   const int rows = (int)c->pending.size();
-  std::vector<double> h((size_t)rows * c->K * NSUMS);
-  HIPCHK(hipMemcpyAsync(h.data(), c->sums, h.size() * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  double* h = c->hsums.data();  // sized for SUM_ROWS rows at creation
+  HIPCHK(hipMemcpyAsync(h, c->sums, (size_t)rows * c->K * NSUMS * sizeof(double),
+                        hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
-  bool bad = false;
   for (const XPending& pr : c->pending)
     for (const auto& w : pr.who) {
       XInst& in = c->inst[w.first];
       const double* s = &h[((size_t)pr.row * c->K + w.first) * NSUMS];
+      bool bad = false;
       for (int k = 0; k < NSUMS; ++k) bad |= !std::isfinite(s[k]);
+      if (bad && !in.nonfinite) {
+        in.nonfinite = true;
+        c->nonfinite_unreported = true;
+      }
       const long long n = w.second, ts = in.p.t_sample;
       if (n % ts == 0) add_sse_record(c, in, n, s[1]);
       if ((n + 1) % ts == 0) add_fe_record(c, in, n + 1, s);
     }
   c->pending.clear();
-  if (bad) return fail(XPFC_ERR_NONFINITE, "non-finite value in a sampled reduction");
+  return XPFC_OK;
+}
+
+// drain, then report instances that diverged since the last report
+int drain_report(xpfc_ctx* c) {
+  int rc = drain(c);
+  if (rc) return rc;
+  if (c->nonfinite_unreported) {
+    c->nonfinite_unreported = false;
+    std::string who;
+    for (int s = 0; s < c->K; ++s)
+      if (c->inst[s].nonfinite && !c->inst[s].nonfinite_reported) {
+        c->inst[s].nonfinite_reported = true;
+        if (who.size() < 200) who += (who.empty() ? "" : " ") + std::to_string(s);
+      }
+    return fail(XPFC_ERR_NONFINITE, "non-finite value in a sampled reduction (instance %s)", who.c_str());
+  }
   return XPFC_OK;
 }
 
@@ -1218,7 +1270,7 @@ extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  return drain(c);
+  return drain_report(c);
 }
 
 extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
@@ -1273,7 +1325,7 @@ extern "C" int xpfc_probe_finite(xpfc_ctx* c, int* finite) {
 extern "C" int xpfc_sync(xpfc_ctx* c) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   HIPCHK(hipStreamSynchronize(c->stream));
-  return drain(c);
+  return drain_report(c);
 }
 
 // counts on success, minus the status code on failure
@@ -1293,7 +1345,10 @@ extern "C" int xpfc_inst_get_samples(xpfc_ctx* c, int s, xpfc_sample* out, int m
   return n;
 }
 extern "C" void xpfc_inst_clear_samples(xpfc_ctx* c, int s) {
-  if (c && s >= 0 && s < c->K && drain(c) == XPFC_OK) c->inst[s].samples.clear();
+  if (c && s >= 0 && s < c->K && drain(c) == XPFC_OK) {
+    c->inst[s].samples.clear();
+    c->inst[s].nonfinite = c->inst[s].nonfinite_reported = false;
+  }
 }
 extern "C" int xpfc_num_samples(xpfc_ctx* c) { return xpfc_inst_num_samples(c, 0); }
 extern "C" int xpfc_get_samples(xpfc_ctx* c, xpfc_sample* out, int max) {
@@ -1555,10 +1610,14 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
     std::fclose(f);
     return fail(XPFC_ERR_PARSE, "%s: not an XPFC checkpoint", path);
   }
+  if (h.params.dim == 2) {  // as xpfc_inst_set_params normalises 2-D params
+    h.params.n[2] = 1;
+    h.params.L[2] = 1.0;
+  }
   if (h.precision != c->prec || h.n0 != c->n0 || h.n1 != c->n1 || h.n2 != c->n2 ||
-      h.stride != c->stride || h.csize != (int64_t)c->csize) {
+      h.stride != c->stride || h.csize != (int64_t)c->csize || !same_grid(c->p, h.params)) {
     std::fclose(f);
-    return fail(XPFC_ERR_ARG, "%s: grid or precision differ from the context's", path);
+    return fail(XPFC_ERR_ARG, "%s: grid, box or precision differ from the context's", path);
   }
   ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
   std::fclose(f);

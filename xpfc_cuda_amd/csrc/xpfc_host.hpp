@@ -51,6 +51,8 @@ struct XInst {
   std::vector<xpfc_sample> samples;
   double last_rmse = 0.0;
   bool have_rmse = false;
+  bool nonfinite = false;           // a sampled sum of this instance was NaN / Inf
+  bool nonfinite_reported = false;  // ... and xpfc_sync has reported it
   double* c2tab = nullptr;  // device C2 table (square / cubic grids)
   int c2cap = 0;            // its capacity in entries
 };
@@ -100,4 +102,6 @@ struct xpfc_ctx {
   std::vector<XInst> inst;
   std::vector<int> active;
   std::vector<XPending> pending;
+  std::vector<double> hsums;         // host rows for drain: [SUM_ROWS][K][NSUMS]
+  bool nonfinite_unreported = false;  // an instance diverged since the last report
 };

@@ -167,7 +167,10 @@ class Service:
     def submit(self, job: Job):
         if job.steps < 1:
             raise ValueError("job %d: steps must be >= 1" % job.jid)
-        self.footprint(job)  # rejects unsupported grids at submission
+        fp = self.footprint(job)  # rejects unsupported grids at submission
+        if fp > self.budget:
+            raise ValueError("job %d: device footprint %d bytes exceeds the budget of %d bytes"
+                             % (job.jid, fp, int(self.budget)))
         self.queues.setdefault((job.dim, job.side), deque()).append(job)
         self.queued += 1
 
@@ -336,6 +339,19 @@ class Service:
         for c in self.classes.values():
             c.close()
         self.classes.clear()
+
+
+def batch_jobs(ninst: int = 64, steps: int = 2000, seed: int = 0,
+               sides=(128, 256, 512, 1024)) -> List[Job]:
+    """The batched workload: `ninst` independent 2-D square-lattice instances,
+    side drawn uniformly from `sides`, sigma ~ U[0.05, 0.2], n0 ~ U[0.02, 0.08],
+    noise amplitude 0.01 seeded by the instance id, `steps` steps each."""
+    rng = np.random.default_rng(seed)
+    side = rng.choice(list(sides), size=ninst)
+    sig = rng.uniform(0.05, 0.2, ninst)
+    n0 = rng.uniform(0.02, 0.08, ninst)
+    return [Job(jid=i, dim=2, side=int(side[i]), sigma=float(sig[i]), n0=float(n0[i]),
+                steps=steps, seed=i) for i in range(ninst)]
 
 
 def log_uniform_side(rng: np.random.Generator, lo: int, hi: int) -> int:
