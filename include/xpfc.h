@@ -268,6 +268,19 @@ int xpfc_inst_set_params(xpfc_ctx* ctx, int s, const xpfc_params* p);
 int xpfc_inst_reset(xpfc_ctx* ctx, int s);
 int xpfc_inst_initialize(xpfc_ctx* ctx, int s);
 int xpfc_inst_initialize_noise(xpfc_ctx* ctx, int s, double amplitude, unsigned long long seed);
+/* xpfc_inst_initialize_noise without any allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate (a serving loop that
+ * admits jobs into resident slots without touching the device or host heap,
+ * harnesses that forbid allocation).  The draws' partial sums go to a fixed
+ * device array of 4096 doubles (32 KiB) that the library holds on each GPU
+ * until the process ends; its device address is looked up at context
+ * creation.  Calls on one GPU from several host threads share the array and
+ * are serialised by the library.  The field is identical to
+ * xpfc_inst_initialize_noise's.  Returns XPFC_ERR_STATE while sampled records
+ * of instance s are pending (read them with xpfc_sync first), and
+ * XPFC_ERR_ARG if the draws would need more partial sums than the array
+ * holds.  On success it calls no hipMalloc / hipFree and no host allocator. */
+int xpfc_inst_initialize_noise_noalloc(xpfc_ctx* ctx, int s, double amplitude, unsigned long long seed);
 int xpfc_inst_set_field(xpfc_ctx* ctx, int s, const double* host_real);
 int xpfc_inst_get_field(xpfc_ctx* ctx, int s, double* host_real);
 long long xpfc_inst_current_step(const xpfc_ctx* ctx, int s);

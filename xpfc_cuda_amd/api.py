@@ -136,6 +136,7 @@ def load_library(path: Optional[str] = None):
         "xpfc_inst_reset": (i, [vp, i]),
         "xpfc_inst_initialize": (i, [vp, i]),
         "xpfc_inst_initialize_noise": (i, [vp, i, d, ct.c_ulonglong]),
+        "xpfc_inst_initialize_noise_noalloc": (i, [vp, i, d, ct.c_ulonglong]),
         "xpfc_inst_set_field": (i, [vp, i, P(d)]),
         "xpfc_inst_get_field": (i, [vp, i, P(d)]),
         "xpfc_inst_current_step": (ll, [vp, i]),
@@ -639,6 +640,11 @@ class BatchContext:
     def initialize_noise(self, s: int, amplitude: float, seed: int):
         _check(lib().xpfc_inst_initialize_noise(self.handle, s, float(amplitude), int(seed)),
                "xpfc_inst_initialize_noise")
+
+    def initialize_noise_noalloc(self, s: int, amplitude: float, seed: int):
+        """initialize_noise with no device or host allocation (same field)."""
+        _check(lib().xpfc_inst_initialize_noise_noalloc(self.handle, s, float(amplitude), int(seed)),
+               "xpfc_inst_initialize_noise_noalloc")
 
     def set_field(self, s: int, psi: np.ndarray):
         a = np.ascontiguousarray(psi, dtype=np.float64)

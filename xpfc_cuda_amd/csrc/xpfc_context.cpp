@@ -335,6 +335,10 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
     return r;
   };
   if (hipSetDevice(p.device) != hipSuccess) return bail(fail(XPFC_ERR_DEVICE, "hipSetDevice(%d) failed", p.device));
+  // the device address of the fixed scratch array of the allocation-free
+  // entry point, looked up once per device with the device current; a failure
+  // only disables that entry point, which reports it
+  (void)noalloc_scratch_setup(p.device);
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
   const size_t sbytes = (size_t)K * c->stride * c->csize;
@@ -758,12 +762,26 @@ extern "C" int xpfc_inst_initialize_vtk(xpfc_ctx* c, int s, const char* path) {
 // psi = psi0 + amplitude * randn(seed, cell) with its mean fixed to psi0, i.e.
 // psi - psi0 = amplitude * (z - mean z): generated on the device with the
 // draws of xpfc_randn, the mean of z summed in a fixed order
+namespace {
+// instance s's field psi0 + amplitude * (z - mean_z) from the draws' mean
+// (both noise entry points); allocates nothing
+int noise_field(xpfc_ctx* c, int s, double amplitude, unsigned long long seed, double mean_z) {
+  if (!std::isfinite(mean_z)) return fail(XPFC_ERR_NONFINITE, "noise draws not finite");
+  c->inst[s].mean = c->inst[s].p.psi0;
+  int rc = upload_inst(c, s);
+  if (rc) return rc;
+  LCHK(launch_noise_fill(c->prec, inst_ptr(c, c->aux, s), c->N, (long long)c->n1 * c->n2, 2 * c->xs, seed,
+                         amplitude, mean_z, c->stream));
+  return field_from_aux(c, s);
+}
+}  // namespace
+
 extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, unsigned long long seed) {
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!std::isfinite(amplitude)) return fail(XPFC_ERR_ARG, "non-finite amplitude");
   if ((rc = drain(c)) != XPFC_OK) return rc;  // pending records use the old mean
-  double* part = c->px + (size_t)s * c->npx_cap * XPART;  // >= 4096 doubles
+  double* part = c->px + (size_t)s * c->npx_cap * XPART;  // >= NOISE_PARTIALS doubles
   int nb = 0;
   LCHK(launch_noise_sum(c->N, seed, part, &nb, c->stream));
   std::vector<double> h(nb);
@@ -771,13 +789,40 @@ extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, 
   HIPCHK(hipStreamSynchronize(c->stream));
   double sz = 0.0;
   for (double v : h) sz += v;
-  const double mean_z = sz / (double)c->N;
-  if (!std::isfinite(mean_z)) return fail(XPFC_ERR_NONFINITE, "noise draws not finite");
-  c->inst[s].mean = c->inst[s].p.psi0;
-  if ((rc = upload_inst(c, s))) return rc;
-  LCHK(launch_noise_fill(c->prec, inst_ptr(c, c->aux, s), c->N, (long long)c->n1 * c->n2, 2 * c->xs, seed,
-                         amplitude, mean_z, c->stream));
-  return field_from_aux(c, s);
+  return noise_field(c, s, amplitude, seed, sz / (double)c->N);
+}
+
+// The same field with no allocation: the draws' partial sums go to the fixed
+// device array of this GPU (its address looked up at context creation) and
+// come back into a stack array; pending records of instance s are not read
+// back (that would grow its sample log), the call refuses instead.
+extern "C" int xpfc_inst_initialize_noise_noalloc(xpfc_ctx* c, int s, double amplitude,
+                                                  unsigned long long seed) {
+  int rc = check_inst(c, s);
+  if (rc) return rc;
+  if (!std::isfinite(amplitude)) return fail(XPFC_ERR_ARG, "non-finite amplitude");
+  for (const XPending& pr : c->pending)
+    for (const auto& w : pr.who)
+      if (w.first == s)
+        return fail(XPFC_ERR_STATE, "sampled records of instance %d are pending: call xpfc_sync first", s);
+  const int nb = noise_partials(c->N);
+  if (nb < 1 || nb > NOISE_PARTIALS)
+    return fail(XPFC_ERR_ARG, "%d partial sums exceed the fixed scratch of %d", nb, NOISE_PARTIALS);
+  double* scr = noalloc_scratch(c->p.device);
+  std::mutex* mu = noalloc_scratch_lock(c->p.device);
+  if (!scr || !mu) return fail(XPFC_ERR_DEVICE, "no scratch array address for device %d", c->p.device);
+  double h[NOISE_PARTIALS];
+  {
+    std::lock_guard<std::mutex> lk(*mu);  // one user of this GPU's array at a time
+    int nbl = 0;
+    LCHK(launch_noise_sum(c->N, seed, scr, &nbl, c->stream));
+    if (nbl != nb) return fail(XPFC_ERR_DEVICE, "noise partial count %d != %d", nbl, nb);
+    HIPCHK(hipMemcpyAsync(h, scr, (size_t)nb * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+  }
+  double sz = 0.0;
+  for (int i = 0; i < nb; ++i) sz += h[i];
+  return noise_field(c, s, amplitude, seed, sz / (double)c->N);
 }
 
 extern "C" int xpfc_initialize(xpfc_ctx* c) { return xpfc_inst_initialize(c, 0); }

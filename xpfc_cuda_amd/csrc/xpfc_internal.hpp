@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <cstddef>
 #include <cstdint>
+#include <mutex>
 
 namespace xpfc {
 
@@ -154,6 +155,8 @@ int launch_unpack(int prec, const void* main_ny, void* natural, int n0n1, int H,
 int launch_to_double(int prec, const void* src, double* dst, long long n, hipStream_t s);
 // noise initial condition: per-workgroup sums of the draws of cells 0..n-1
 // (nblocks returned, <= 4096), then out[c] = amp * (z_c - mean_z) as reals
+constexpr int NOISE_PARTIALS = 4096;  // most per-workgroup sums of launch_noise_sum
+int noise_partials(long long n);     // the number launch_noise_sum produces for n cells
 int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s);
 // (the reals of cell c land at (c / plane) * pitch + c % plane)
 int launch_noise_fill(int prec, void* out, long long n, long long plane, long long pitch,
@@ -162,5 +165,13 @@ int launch_noise_fill(int prec, void* out, long long n, long long plane, long lo
 int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
                  hipStream_t s);
 int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s);
+// Fixed device array (NOISE_PARTIALS doubles per GPU, held until the process
+// ends) for the allocation-free noise initialisation.  setup: looks up the
+// array's device address with `device` current (once per device; 0, or < 0
+// on failure); noalloc_scratch: that address (nullptr before setup); the lock
+// serialises the host threads that use one device's array.
+int noalloc_scratch_setup(int device);
+double* noalloc_scratch(int device);
+std::mutex* noalloc_scratch_lock(int device);
 
 }  // namespace xpfc

@@ -4,6 +4,7 @@
 #include "xpfc_device.hpp"
 
 #include <algorithm>
+#include <mutex>
 
 namespace xpfc {
 
@@ -191,8 +192,12 @@ __global__ __launch_bounds__(256) void k_noise_fill(T* __restrict__ out, long lo
   }
 }
 
+int noise_partials(long long n) {
+  return (int)std::min<long long>(NOISE_PARTIALS, std::max<long long>(1, (n + 255) / 256));
+}
+
 int launch_noise_sum(long long n, unsigned long long seed, double* partial, int* nblocks, hipStream_t s) {
-  const int nb = (int)std::min<long long>(4096, std::max<long long>(1, (n + 255) / 256));
+  const int nb = noise_partials(n);
   *nblocks = nb;
   hipLaunchKernelGGL(k_noise_sum, dim3(nb), dim3(256), 0, s, n, (uint32_t)seed, (uint32_t)(seed >> 32),
                      partial);
@@ -209,6 +214,47 @@ int launch_noise_fill(int prec, void* out, long long n, long long plane, long lo
     hipLaunchKernelGGL(k_noise_fill<double>, dim3(nb), dim3(256), 0, s, (double*)out, n, plane, pitch,
                        (uint32_t)seed, (uint32_t)(seed >> 32), amp, mean_z);
   return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free noise initialisation: the per-workgroup
+// partial sums of k_noise_sum (at most NOISE_PARTIALS of them) live in this
+// fixed device array instead of a context buffer.  Each GPU holds its own copy
+// at its own address until the process ends; host code never passes the
+// array's name to a kernel (on the host it is only a stand-in for the symbol)
+// but the device address looked up once per device by noalloc_scratch_setup.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) double scratch[NOISE_PARTIALS];
+
+namespace {
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mu;                     // guards the table below
+double* g_scratch_ptr[MAX_DEVICES] = {};     // device address of `scratch`, per device
+std::mutex g_scratch_use[MAX_DEVICES];       // one user of a device's array at a time
+}  // namespace
+
+int noalloc_scratch_setup(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return -1;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return 0;
+  void* p = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !p) {
+    (void)hipGetLastError();
+    return -2;
+  }
+  g_scratch_ptr[device] = static_cast<double*>(p);
+  return 0;
+}
+
+double* noalloc_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch_ptr[device];
+}
+
+std::mutex* noalloc_scratch_lock(int device) {
+  return (device >= 0 && device < MAX_DEVICES) ? &g_scratch_use[device] : nullptr;
 }
 
 // Divergence probe: one workgroup; flag[s] = 1 when two modes of instance s
