@@ -49,10 +49,12 @@ def spectrum_bytes(n, prec):
     return n * n * (n // 2 + 1) * csize
 
 
-def gpu_run(n, prec, steps, warmup, seed=2):
+def gpu_run(n, prec, steps, warmup, seed=2, pin=None):
     import xpfc_cuda_amd as X
     p = fcc_params(n, prec)
     with X.Context(p) as ctx:
+        if pin:
+            ctx.set_variants(*pin)   # no autotuning
         ctx.initialize_noise(0.01, seed)
         if warmup:
             ctx.step(warmup)
@@ -256,6 +258,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-configs", action="store_true",
+                    help="skip the 2-D 256^2 and 3-D 256^3 FP64 configuration runs")
     ap.add_argument("--serve", type=float, default=30.0,
                     help="seconds of Poisson arrivals for the serving leg (0 = skip)")
     ap.add_argument("--serve-rate", type=float, default=20.0)
@@ -269,11 +273,16 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run --steps unsampled steps and one field read-back (its D2D copy of "
                          "the packed spectrum is the known-size copy for counter calibration)")
+    ap.add_argument("--variants", default="",
+                    help="pass variants 'x,y,z' of the 512^3 FP32 run (and the traffic probe) "
+                         "to pin instead of autotuning")
     args = ap.parse_args()
     if args.traffic_probe:
         import xpfc_cuda_amd as X
         p = fcc_params(args.n, 1, t_sample=0)
         with X.Context(p) as ctx:
+            if args.variants:
+                ctx.set_variants(*[int(v) for v in args.variants.split(",")])
             ctx.initialize_noise(0.01, 2)
             ctx.step(args.steps)
             ctx.get_field()
@@ -298,7 +307,8 @@ def main():
                                                   lookahead=args.serve_lookahead)}), flush=True)
         return
     n = args.n
-    ms, wall, passes, samples, variants, io = gpu_run(n, 1, args.steps, args.warmup)
+    pin = [int(v) for v in args.variants.split(",")] if args.variants else None
+    ms, wall, passes, samples, variants, io = gpu_run(n, 1, args.steps, args.warmup, pin=pin)
     ms_per_step = ms / args.steps
     value = 1000.0 / ms_per_step
     S = spectrum_bytes(n, 1)
@@ -358,7 +368,8 @@ def main():
             tr = json.load(f)["k_xstep"].get(str(variants[0]))
         if tr:
             out["roofline"]["traffic"] = tr["traffic_bytes"]
-    out["configs"] = config_runs()
+    if not args.no_configs:
+        out["configs"] = config_runs()
     if not args.no_batch:
         out["batched"] = batched_run()
     if args.serve > 0:

@@ -19,8 +19,11 @@ BINDIR = os.path.join(HERE, "bin")
 OBJDIR = os.path.join(HERE, "_build")
 
 ARCH = os.environ.get("XPFC_OFFLOAD_ARCH", "gfx950")
-# packed FP32 instructions stay off (they are generated by -O3 otherwise)
-HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
+# packed FP32 instructions stay off (they are generated by -O3 otherwise);
+# multiply-adds are fused only within one source expression (-ffp-contract=on),
+# so the rounding of a line's arithmetic does not depend on the code around it
+# (pass variants that restructure a kernel give bitwise-identical fields)
+HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=on",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include")]
 

@@ -59,9 +59,13 @@ __device__ __forceinline__ int xcd_tile(int b, int n, int lgc) {
   return xcd < rem ? xcd * (per + 1) + slot : rem * (per + 1) + (xcd - rem) * per + slot;
 }
 
+// Where a tile's column lies: its segment's row stride and kind are copied
+// out (no pointer into the kernel arguments, which would force a private copy
+// of them once the location is carried across loop iterations).
 struct TileLoc {
-  const Seg* sg;
   int col;
+  int rs;
+  int kind;
   bool act;
   long long off;
 };
@@ -70,11 +74,13 @@ __device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, in
   TileLoc r;
   const bool second = tile >= segs.tiles0;
   if (second) tile -= segs.tiles0;
-  r.sg = second ? &segs.s[1] : &segs.s[0];
-  const int b = tile / r.sg->ntc, ct = tile - b * r.sg->ntc;
+  const Seg& sg = second ? segs.s[1] : segs.s[0];
+  const int b = tile / sg.ntc, ct = tile - b * sg.ntc;
   r.col = ct * TC + ci;
-  r.act = r.col < r.sg->ncols;
-  r.off = r.sg->off + (long long)b * r.sg->bs + (long long)r.col * r.sg->cs;
+  r.act = r.col < sg.ncols;
+  r.off = sg.off + (long long)b * sg.bs + (long long)r.col * sg.cs;
+  r.rs = sg.rs;
+  r.kind = sg.kind;
   return r;
 }
 
@@ -97,7 +103,7 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   buf += (long long)bt.slots[blockIdx.y] * bt.stride;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ci, TC);
-  const int rs = loc.sg->rs;
+  const int rs = loc.rs;
   C* p = buf + loc.off;
   C v[E];
 #pragma unroll
@@ -124,40 +130,60 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
 // and C2 from the instance's table over m (no FP64 exp in the kernel, which
 // keeps the register footprint of the update small); otherwise k^2 and C2 are
 // evaluated from the per-axis wavenumbers.
-template <typename T, int L, int TC, bool SAMPLE, bool LUT>
-__global__ __launch_bounds__((tile_threads<L, TC>()))
-void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
-             const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
+//
+// Measured (512^3 FP32, rocprofv3 counters): this pass keeps fewer reads in
+// flight than the one-stream passes and sees almost no DRAM credit stalls, so
+// it is bound by how many requests it issues, not by HBM.  Raising occupancy
+// by capping registers (80 or 64 VGPRs) spills and runs 20-150 % slower;
+// walking 2 or 4 tiles per workgroup with the next tile's loads in flight
+// needs 167-178 VGPRs; fetching the state or C2 after the forward transform
+// is 4 % slower.  The variant below (one tile, both streams and C2 loaded up
+// front) is the fastest measured.
+
+// One tile's loads: the nonlinear term and the state along the column.
+template <typename T, int L, int E>
+struct XTile {
+  cpx<T> v[E], ps[E];
+  TileLoc loc;
+};
+
+template <typename T, int L, int TC>
+__device__ __forceinline__ void xload(XTile<T, L, elems_per_thread<L>()>& x, const Segs& segs, int tile,
+                                      int ntiles, int ci, int t, const cpx<T>* __restrict__ work,
+                                      const cpx<T>* __restrict__ psi) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
-  constexpr int NT = tile_threads<L, TC>();
-  __shared__ C lds[L * (TC + PADT)];
-  __shared__ C twl[L];
-  const int slot = bt.slots[blockIdx.y];
-  work += (long long)slot * bt.stride;
-  psi += (long long)slot * bt.stride;
-  partial += (long long)slot * bt.pstride;
-  const Phys& ph = bt.ia[slot].ph;
-  const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
-  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ci, TC);
-  const int rs = loc.sg->rs;
+  x.loc = locate(segs, tile < ntiles ? tile : 0, ci, TC);
+  x.loc.act = x.loc.act && tile < ntiles;
+  const int rs = x.loc.rs;
+  const C* pw = work + x.loc.off;
+  const C* pp = psi + x.loc.off;
+#pragma unroll
+  for (int m = 0; m < E; ++m) x.v[m] = x.loc.act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+#pragma unroll
+  for (int m = 0; m < E; ++m) x.ps[m] = x.loc.act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+}
+
+template <typename T, int L, int TC, bool SAMPLE, bool LUT>
+__device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int ci, int t,
+                                      cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi,
+                                      cpx<T>* lds, const cpx<T>* twl, const KGeom& g, const Phys& ph,
+                                      double (&acc)[XPART]) {
+  using C = cpx<T>;
+  constexpr int E = elems_per_thread<L>();
+  const TileLoc& loc = x.loc;
+  const int rs = loc.rs;
   const bool act = loc.act;
   C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
   // wavenumber of this column
   int kb, kz;
-  if (loc.sg->kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
+  if (loc.kind == 0) { kb = loc.col / g.H; kz = loc.col - kb * g.H; }
   else { kb = loc.col; kz = g.H; }
   const int jb = signed_index(kb, g.n1);
   const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
-  // both input streams are read up front (their latency hides behind the
-  // forward transform), the two output streams written in bursts after it
-  C v[E], ps[E];
-#pragma unroll
-  for (int m = 0; m < E; ++m) v[m] = act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
-#pragma unroll
-  for (int m = 0; m < E; ++m) ps[m] = act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  // the line's C2 values are gathered with the input streams
   double c2t[LUT ? E : 1];
   if constexpr (LUT) {
 #pragma unroll
@@ -166,14 +192,13 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
       c2t[m] = c2_lookup(ia * ia + myz, ph);
     }
   }
-  stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
+  C (&v)[E] = x.v;
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
 
   const double ky = g.dk1 * (double)jb;
   const double kzv = g.dk2 * (double)kz;
   const double kyz2 = ky * ky + kzv * kzv;
-  const double wgt = act ? ((loc.sg->kind == 0 && kz == 0) || loc.sg->kind == 1 ? 1.0 : 2.0) : 0.0;
-  double acc[XPART] = {0.0, 0.0};
+  const double wgt = act ? ((loc.kind == 0 && kz == 0) || loc.kind == 1 ? 1.0 : 2.0) : 0.0;
 
   // psi_k' = (psi_k - dt M k^2 N_k) / den  (euler_step.cu:50-59).  The
   // denominator and its reciprocal are FP64; the numerator is field arithmetic
@@ -184,7 +209,7 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   for (int m = 0; m < E; ++m) {
     __builtin_amdgcn_sched_barrier(0);
     const int ka = t + m * (L / E);
-    const C old = ps[m];
+    const C old = x.ps[m];
     const int ia = signed_index(ka, g.n0);
     double k2, c2;
     if constexpr (LUT) {
@@ -228,6 +253,31 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
 #pragma unroll
     for (int m = 0; m < E; ++m) pw[(t + m * (L / E)) * rs] = v[m];
   }
+}
+
+template <typename T, int L, int TC, bool SAMPLE, bool LUT>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
+void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
+             const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
+  using C = cpx<T>;
+  constexpr int E = elems_per_thread<L>();
+  constexpr int NT = tile_threads<L, TC>();
+  __shared__ C lds[L * (TC + PADT)];
+  __shared__ C twl[L];
+  const int slot = bt.slots[blockIdx.y];
+  work += (long long)slot * bt.stride;
+  psi += (long long)slot * bt.stride;
+  partial += (long long)slot * bt.pstride;
+  const Phys& ph = bt.ia[slot].ph;
+  const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
+  const int ntiles = segs.tiles0 + segs.tiles1;
+  double acc[XPART] = {0.0, 0.0};
+  // both input streams are read up front (their latency hides behind the
+  // forward transform), the two output streams written in bursts after it
+  XTile<T, L, E> x;
+  xload<T, L, TC>(x, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ntiles, ci, t, work, psi);
+  stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
+  xtile<T, L, TC, SAMPLE, LUT>(x, ci, t, work, psi, lds, twl, g, ph, acc);
   if constexpr (SAMPLE) block_sum<NT, XPART>(acc, partial + (long long)blockIdx.x * XPART);
 }
 
@@ -628,7 +678,8 @@ int strided_blocks(int prec, int L, int tc, const Segs& segs) {
   return prec == 1 ? blocks_T<float>(L, tc, segs) : blocks_T<double>(L, tc, segs);
 }
 
-// tc: tile width; the square / cubic table path is taken when g.cubic is set
+// tc: tile width; the square / cubic table path is taken when g.cubic is set.
+// Returns the workgroup count (= number of X partials).
 template <typename T>
 static int xstep_T(void* work, void* psi, const Segs& segs0, int L, int tc, const void* tw,
                    const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
