@@ -292,6 +292,26 @@ int xpfc_inst_avg_calc(xpfc_ctx* ctx, int s, double* mean);
 int xpfc_inst_sse_calc(xpfc_ctx* ctx, int s, double* rmse);
 int xpfc_inst_write_vtk(xpfc_ctx* ctx, int s, const char* path);
 
+/* Parameters of instance s (any context). */
+int xpfc_get_inst_params(const xpfc_ctx* ctx, int s, xpfc_params* out);
+
+/* ---- simulations of several grids in one context --------------------------- */
+/* No counterpart in the reference.  The instances may have different grids
+ * (2-D and 3-D, any supported extents and box lengths) but one precision and
+ * device.  Every step advances all active instances of every grid with one
+ * launch per pass (the axis-1 passes cover the 3-D grids), and runs of
+ * unsampled steps are replayed from captured hipGraphs as in one-grid
+ * contexts, so a block of steps of the whole mix is one graph.  Each
+ * instance's field is bitwise the one a one-instance context gives; sampled
+ * FE / RMSE records sum their partials in another grouping (equal to
+ * rounding), and xpfc_inst_fe_calc is bitwise the same.  The xpfc_inst_*
+ * functions, xpfc_set_active (instance numbers of this context), xpfc_step,
+ * xpfc_sync, xpfc_query, xpfc_probe_finite, xpfc_time_steps and the functions
+ * on instance 0 accept it; the operator-level functions, xpfc_set_variants and
+ * xpfc_time_passes refuse it (XPFC_ERR_ARG); xpfc_autotune has nothing to
+ * choose (variants 0). */
+int xpfc_create_mixed(const xpfc_params* p, int ninst, xpfc_ctx** out);  /* p[ninst] */
+
 /* ---- errors ----------------------------------------------------------------- */
 const char* xpfc_strerror(int status);
 const char* xpfc_last_error(void);   /* detail of the last failure on this thread */

@@ -1,0 +1,135 @@
+"""Contexts of several grids (one launch per pass over every grid): each
+instance evolves exactly as it would alone, bit for bit, whatever the other
+grids, physics, sampling periods, active lists and step counts are.
+
+Sampled FE / RMSE records sum per-workgroup partials whose grouping follows
+the launch shape, so they are compared to rounding; fe_calc on demand uses a
+fixed shape and is compared bitwise.
+"""
+import numpy as np
+import pytest
+
+from xpfc_cuda_amd import Context, MixedContext, XPFC_FP32_MIXED, XPFC_FP64
+
+from cases import growth_case, make
+
+pytestmark = pytest.mark.gpu
+
+
+def _flush(msg):
+    print(msg, flush=True)
+
+
+def _mix(prec):
+    """Seven instances on four grids (2-D and 3-D, one non-square box), with
+    different physics and sampling periods; the 64^2 grid holds three."""
+    ps = []
+    _, p = growth_case((64, 64), prec=prec, t_sample=7)
+    ps.append(p)
+    _, p = make((16, 16, 32), prec=prec, sigma=0.1, psi0=0.02, t_sample=5, dt=0.3)
+    ps.append(p)
+    _, p = make((64, 64), prec=prec, sigma=0.2, psi0=0.05, eps=0.01, mode="sum", t_sample=0)
+    ps.append(p)
+    _, p = make((32, 64), dx=0.1, prec=prec, sigma=0.08, psi0=0.04, Mo=1.5, int_Mo=0, t_sample=10)
+    ps.append(p)
+    _, p = make((32, 32, 32), prec=prec, sigma=0.09, psi0=0.03, t_sample=4, powf_fp32=1)
+    ps.append(p)
+    _, p = make((64, 64), prec=prec, sigma=0.07, psi0=0.05, t_sample=9)
+    ps.append(p)
+    _, p = make((16, 16, 32), prec=prec, sigma=0.12, psi0=0.06, t_sample=0)
+    ps.append(p)
+    return ps
+
+
+def _field(shape, seed):
+    return 0.05 + 0.02 * np.random.default_rng(seed).standard_normal(shape)
+
+
+def _alone(p, psi, nsteps):
+    with Context(p) as c:
+        c.set_field(psi)
+        c.step(nsteps)
+        return c.get_field(), c.samples(), c.fe_calc()
+
+
+def _close_samples(a, b, rel):
+    assert [(x.step, x.has_fe, x.has_sse) for x in a] == [(y.step, y.has_fe, y.has_sse) for y in b]
+    for x, y in zip(a, b):
+        for f in ("F_ref", "f_mean", "mean", "rmse"):
+            u, v = getattr(x, f), getattr(y, f)
+            assert (np.isnan(u) and np.isnan(v)) or abs(u - v) <= rel * abs(v) + 1e-300, (f, x.step, u, v)
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_mixed_equals_alone(prec, graphs):
+    _flush("mixed vs alone prec=%d graphs=%d" % (prec, graphs))
+    ps = _mix(prec)
+    psis = [_field(tuple(p.n), 30 + s) for s, p in enumerate(ps)]
+    nsteps = 61
+    with MixedContext(ps) as mc:
+        mc.set_graphs(graphs)
+        for s, x in enumerate(psis):
+            mc.set_field(s, x)
+        mc.step(nsteps)
+        mc.sync()
+        got = [(mc.get_field(s), mc.samples(s), mc.fe_calc(s), mc.current_step(s)) for s in range(len(ps))]
+    for s, p in enumerate(ps):
+        f, smp, fe = _alone(p, psis[s], nsteps)
+        assert np.all(np.isfinite(f))
+        assert got[s][3] == nsteps
+        assert np.array_equal(got[s][0], f), "instance %d (%s) differs" % (s, p.n)
+        assert got[s][2] == fe
+        _close_samples(got[s][1], smp, 1e-12 if prec == XPFC_FP64 else 1e-9)
+    assert not np.array_equal(got[0][0], got[2][0])  # same grid, own physics
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+def test_mixed_active_subsets_and_slot_reuse(prec):
+    _flush("mixed active subsets prec=%d" % prec)
+    ps = _mix(prec)
+    psis = [_field(tuple(p.n), 50 + s) for s, p in enumerate(ps)]
+    with MixedContext(ps) as mc:
+        for s, x in enumerate(psis):
+            mc.set_field(s, x)
+        mc.set_active([4, 0, 3])          # three grids, instances listed out of order
+        mc.step(12)
+        mc.set_active([1])                # one 3-D grid alone
+        mc.step(5)
+        mc.set_active(None)
+        mc.step(8)
+        steps = [mc.current_step(s) for s in range(len(ps))]
+        f = [mc.get_field(s) for s in range(len(ps))]
+        # slot 5 handed to a new job: new physics and field, step counter reset
+        _, pnew = make((64, 64), prec=prec, sigma=0.09, psi0=0.03, t_sample=3)
+        xnew = _field((64, 64), 99)
+        mc.set_params(5, pnew)
+        mc.reset(5)
+        mc.set_field(5, xnew)
+        mc.step(6)
+        f5 = mc.get_field(5)
+        f0b = mc.get_field(0)
+    assert steps == [20, 13, 8, 20, 20, 8, 8]
+    for s, p in enumerate(ps):
+        a, _, _ = _alone(p, psis[s], steps[s])
+        assert np.array_equal(f[s], a), "instance %d differs" % s
+    a5, _, _ = _alone(pnew, xnew, 6)
+    assert np.array_equal(f5, a5)
+    a0, _, _ = _alone(ps[0], psis[0], 26)
+    assert np.array_equal(f0b, a0)
+
+
+def test_mixed_rejects_operator_calls_and_mixed_precision():
+    from xpfc_cuda_amd import XPFCError
+    _, a = make((32, 32), prec=XPFC_FP64)
+    _, b = make((16, 16, 16), prec=XPFC_FP32_MIXED)
+    with pytest.raises(XPFCError):
+        MixedContext([a, b])
+    _, b = make((16, 16, 16), prec=XPFC_FP64)
+    with MixedContext([a, b]) as mc:
+        with pytest.raises(XPFCError):
+            mc.set_variants(8, 8, 256)
+        assert mc.autotune() == (0, 0, 0)
+        mc.initialize_noise(0, 0.01, 1)
+        with pytest.raises(XPFCError):
+            mc.step(1)                    # instance 1 has no field yet

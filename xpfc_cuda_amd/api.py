@@ -127,6 +127,8 @@ def load_library(path: Optional[str] = None):
         "xpfc_save_checkpoint": (i, [vp, ct.c_char_p]),
         "xpfc_load_checkpoint": (i, [vp, ct.c_char_p]),
         "xpfc_create_batch": (i, [P(_Params), i, P(vp)]),
+        "xpfc_create_mixed": (i, [P(_Params), i, P(vp)]),
+        "xpfc_get_inst_params": (i, [vp, i, P(_Params)]),
         "xpfc_num_instances": (i, [vp]),
         "xpfc_instance_bytes": (sz, [P(_Params)]),
         "xpfc_query": (i, [vp]),
@@ -646,13 +648,16 @@ class BatchContext:
         _check(lib().xpfc_inst_initialize_noise_noalloc(self.handle, s, float(amplitude), int(seed)),
                "xpfc_inst_initialize_noise_noalloc")
 
+    def shape_of(self, s: int):
+        return self.shape
+
     def set_field(self, s: int, psi: np.ndarray):
         a = np.ascontiguousarray(psi, dtype=np.float64)
-        assert a.shape == self.shape, (a.shape, self.shape)
+        assert a.shape == self.shape_of(s), (a.shape, self.shape_of(s))
         _check(lib().xpfc_inst_set_field(self.handle, s, _dptr(a)), "xpfc_inst_set_field")
 
     def get_field(self, s: int) -> np.ndarray:
-        out = np.empty(self.shape, dtype=np.float64)
+        out = np.empty(self.shape_of(s), dtype=np.float64)
         _check(lib().xpfc_inst_get_field(self.handle, s, _dptr(out)), "xpfc_inst_get_field")
         return out
 
@@ -703,3 +708,29 @@ class BatchContext:
     def load_checkpoint(self, s: int, path: str):
         _check(lib().xpfc_inst_load_checkpoint(self.handle, s, path.encode()),
                "xpfc_inst_load_checkpoint")
+
+
+class MixedContext(BatchContext):
+    """Simulations of several grids in one context (xpfc_create_mixed): each
+    step advances every active instance of every grid with one launch per
+    pass, and runs of unsampled steps replay captured graphs of those launches.
+    Instance s keeps the grid of params[s]; its field equals the one a
+    one-instance context gives, bit for bit.  No counterpart in the reference
+    (one simulation per process, main.cu:20-435)."""
+
+    def __init__(self, params: Sequence[Params]):
+        if not params:
+            raise ValueError("at least one instance")
+        self.params = list(params)
+        arr = (_Params * len(self.params))(*[q.to_c() for q in self.params])
+        h = ct.c_void_p()
+        _check(lib().xpfc_create_mixed(arr, len(self.params), ct.byref(h)), "xpfc_create_mixed")
+        self.handle = h
+
+    @property
+    def shape(self):
+        raise AttributeError("a mixed context has one shape per instance: use shape_of(s)")
+
+    def shape_of(self, s: int):
+        return tuple(self.params[s].n)
+

@@ -22,6 +22,25 @@
 #include <thread>
 #include <vector>
 
+// Mixed-grid contexts: an instance call goes to the class context that holds
+// the instance, with its slot there (named `cc` and `ss` in `expr`).
+#define XPFC_ROUTE(c, s, expr)                                                            \
+  do {                                                                                    \
+    if ((c) && !(c)->cls.empty()) {                                                       \
+      if ((s) < 0 || (s) >= (c)->K)                                                       \
+        return fail(XPFC_ERR_ARG, "instance %d out of range [0, %d)", (int)(s), (c)->K);  \
+      xpfc_ctx* const cc = (c)->cls[(size_t)(c)->where[(size_t)(s)].first];              \
+      const int ss = (c)->where[(size_t)(s)].second;                                      \
+      return expr;                                                                        \
+    }                                                                                     \
+  } while (0)
+// Context-level operations a mixed-grid context does not offer.
+#define XPFC_NOT_MIXED(c, what)                                                             \
+  do {                                                                                      \
+    if ((c) && !(c)->cls.empty())                                                           \
+      return fail(XPFC_ERR_ARG, "%s is not available for a context of several grids", what); \
+  } while (0)
+
 using namespace xpfc;
 
 namespace {
@@ -148,13 +167,23 @@ void drop_graphs(xpfc_ctx* c) {
 
 void free_all(xpfc_ctx* c) {
   drop_graphs(c);
+  for (auto& g : c->rgraphs) (void)hipGraphExecDestroy(g.second);
+  c->rgraphs.clear();
+  for (auto& t : c->rtab)
+    if (t.second.d) (void)hipFree(t.second.d);
+  c->rtab.clear();
+  for (xpfc_ctx* k : c->cls) {
+    free_all(k);
+    delete k;
+  }
+  c->cls.clear();
   void* ptrs[] = {c->psi, c->work, c->aux, c->tw0, c->tw1, c->twz, c->twp, c->px,
-                  c->pz, c->sums, c->d_ia, c->d_ids, c->d_active, c->d_flags};
+                  c->pz, c->sums, c->d_ia, c->d_ids, c->d_active, c->d_flags, c->d_budget};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
   for (XInst& in : c->inst)
     if (in.c2tab) (void)hipFree(in.c2tab);
-  if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream && !c->borrowed_stream) (void)hipStreamDestroy(c->stream);
   if (c->stage) (void)hipHostFree(c->stage);
 }
 
@@ -267,6 +296,12 @@ int drain_report(xpfc_ctx* c);
 // (defined with the field I/O).
 int field_from_aux(xpfc_ctx* c, int s);
 
+// Contexts of several grids (defined at the end of the file).
+int mixed_step(xpfc_ctx* c, int nsteps);
+int mixed_set_active(xpfc_ctx* c, const int* slots, int n);
+int mixed_sync(xpfc_ctx* c);
+int mixed_probe(xpfc_ctx* c, int* finite);
+
 int check_inst(const xpfc_ctx* c, int s) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (s < 0 || s >= c->K) return fail(XPFC_ERR_ARG, "instance %d out of range [0, %d)", s, c->K);
@@ -278,7 +313,10 @@ int check_inst(const xpfc_ctx* c, int s) {
 // ---------------------------------------------------------------------------
 // creation
 // ---------------------------------------------------------------------------
-extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
+namespace {
+// A context of K instances of one grid on its own stream, or (shared != null)
+// on the stream of the mixed-grid context it is a class of.
+int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out) {
   if (!ps || !out || K < 1) return fail(XPFC_ERR_ARG, "bad arguments");
   *out = nullptr;
   xpfc_params p = ps[0];
@@ -339,8 +377,12 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   // entry point, looked up once per device with the device current; a failure
   // only disables that entry point, which reports it
   (void)noalloc_scratch_setup(p.device);
-  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+  if (shared) {
+    c->stream = shared;
+    c->borrowed_stream = true;
+  } else if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
     return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
+  }
   const size_t sbytes = (size_t)K * c->stride * c->csize;
   // the runtime may grant allocations beyond the device's memory and fail
   // later; refuse what cannot fit up front
@@ -380,7 +422,8 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
       hipMalloc((void**)&c->d_ia, (size_t)(K + 1) * sizeof(InstArgs)) != hipSuccess ||
       hipMalloc((void**)&c->d_ids, (size_t)(K + 1) * sizeof(int)) != hipSuccess ||
       hipMalloc((void**)&c->d_active, (size_t)K * sizeof(int)) != hipSuccess ||
-      hipMalloc((void**)&c->d_flags, (size_t)K * sizeof(int)) != hipSuccess)
+      hipMalloc((void**)&c->d_flags, (size_t)K * sizeof(int)) != hipSuccess ||
+      hipMalloc((void**)&c->d_budget, (size_t)K * sizeof(int)) != hipSuccess)
     return bail(fail(XPFC_ERR_NOMEM, "table allocation failed"));
   try {
     c->hsums.assign((size_t)SUM_ROWS * K * NSUMS, 0.0);  // read-back rows of drain
@@ -418,6 +461,11 @@ extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
   c->use_graphs = !(eg && eg[0] == '0');
   *out = c;
   return XPFC_OK;
+}
+}  // namespace
+
+extern "C" int xpfc_create_batch(const xpfc_params* ps, int K, xpfc_ctx** out) {
+  return create_impl(ps, K, nullptr, out);
 }
 
 extern "C" int xpfc_create(const xpfc_params* p, xpfc_ctx** out) { return xpfc_create_batch(p, 1, out); }
@@ -462,18 +510,34 @@ extern "C" void xpfc_destroy(xpfc_ctx* c) {
 
 extern "C" int xpfc_get_params(const xpfc_ctx* c, xpfc_params* out) {
   if (!c || !out) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->cls.empty()) return xpfc_get_inst_params(c, 0, out);
   *out = c->inst[0].p;
   return XPFC_OK;
 }
 
 extern "C" int xpfc_num_instances(const xpfc_ctx* c) { return c ? c->K : 0; }
-extern "C" long long xpfc_num_cells(const xpfc_ctx* c) { return c ? c->N : 0; }
-extern "C" size_t xpfc_real_bytes(const xpfc_ctx* c) { return c ? (size_t)c->N * c->rsize : 0; }
+// (mixed-grid contexts: those of instance 0's grid)
+extern "C" long long xpfc_num_cells(const xpfc_ctx* c) {
+  if (c && !c->cls.empty()) return xpfc_num_cells(c->cls[c->where[0].first]);
+  return c ? c->N : 0;
+}
+extern "C" size_t xpfc_real_bytes(const xpfc_ctx* c) {
+  if (c && !c->cls.empty()) return xpfc_real_bytes(c->cls[c->where[0].first]);
+  return c ? (size_t)c->N * c->rsize : 0;
+}
 extern "C" size_t xpfc_spectrum_bytes(const xpfc_ctx* c) {
+  if (c && !c->cls.empty()) return xpfc_spectrum_bytes(c->cls[c->where[0].first]);
   return c ? (size_t)c->n0 * c->n1 * (c->H + 1) * c->csize : 0;
+}
+extern "C" int xpfc_get_inst_params(const xpfc_ctx* c, int s, xpfc_params* out) {
+  if (!c || !out || s < 0 || s >= c->K) return fail(XPFC_ERR_ARG, "bad arguments");
+  if (!c->cls.empty()) return xpfc_get_inst_params(c->cls[c->where[s].first], c->where[s].second, out);
+  *out = c->inst[s].p;
+  return XPFC_OK;
 }
 
 extern "C" int xpfc_inst_set_params(xpfc_ctx* c, int s, const xpfc_params* p) {
+  XPFC_ROUTE(c, s, xpfc_inst_set_params(cc, ss, p));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!p) return fail(XPFC_ERR_ARG, "null params");
@@ -489,6 +553,7 @@ extern "C" int xpfc_inst_set_params(xpfc_ctx* c, int s, const xpfc_params* p) {
 }
 
 extern "C" int xpfc_inst_reset(xpfc_ctx* c, int s) {
+  XPFC_ROUTE(c, s, xpfc_inst_reset(cc, ss));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if ((rc = drain(c)) != XPFC_OK) return rc;
@@ -503,6 +568,7 @@ extern "C" int xpfc_inst_reset(xpfc_ctx* c, int s) {
 
 extern "C" int xpfc_set_active(xpfc_ctx* c, const int* slots, int n) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!c->cls.empty()) return mixed_set_active(c, slots, n);
   std::vector<int> a;
   if (!slots) {
     for (int s = 0; s < c->K; ++s) a.push_back(s);
@@ -609,6 +675,7 @@ int upload_fluct(xpfc_ctx* c, int s, const double* host, double mean) {
 }  // namespace
 
 extern "C" int xpfc_inst_set_field(xpfc_ctx* c, int s, const double* host) {
+  XPFC_ROUTE(c, s, xpfc_inst_set_field(cc, ss, host));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!host) return fail(XPFC_ERR_ARG, "null field");
@@ -650,6 +717,7 @@ int field_from_aux(xpfc_ctx* c, int s) {
 }  // namespace
 
 extern "C" int xpfc_inst_get_field(xpfc_ctx* c, int s, double* host) {
+  XPFC_ROUTE(c, s, xpfc_inst_get_field(cc, ss, host));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!host) return fail(XPFC_ERR_ARG, "null field");
@@ -683,6 +751,7 @@ extern "C" int xpfc_get_field(xpfc_ctx* c, double* host) { return xpfc_inst_get_
 // initial conditions (initialize.cu:95-202)
 // ---------------------------------------------------------------------------
 extern "C" int xpfc_inst_initialize(xpfc_ctx* c, int s) {
+  XPFC_ROUTE(c, s, xpfc_inst_initialize(cc, ss));
   int rc = check_inst(c, s);
   if (rc) return rc;
   const xpfc_params& p = c->inst[s].p;
@@ -731,6 +800,7 @@ extern "C" int xpfc_inst_initialize(xpfc_ctx* c, int s) {
 }
 
 extern "C" int xpfc_inst_initialize_vtk(xpfc_ctx* c, int s, const char* path) {
+  XPFC_ROUTE(c, s, xpfc_inst_initialize_vtk(cc, ss, path));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!path) return fail(XPFC_ERR_ARG, "null path");
@@ -777,6 +847,7 @@ int noise_field(xpfc_ctx* c, int s, double amplitude, unsigned long long seed, d
 }  // namespace
 
 extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, unsigned long long seed) {
+  XPFC_ROUTE(c, s, xpfc_inst_initialize_noise(cc, ss, amplitude, seed));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!std::isfinite(amplitude)) return fail(XPFC_ERR_ARG, "non-finite amplitude");
@@ -798,6 +869,7 @@ extern "C" int xpfc_inst_initialize_noise(xpfc_ctx* c, int s, double amplitude, 
 // back (that would grow its sample log), the call refuses instead.
 extern "C" int xpfc_inst_initialize_noise_noalloc(xpfc_ctx* c, int s, double amplitude,
                                                   unsigned long long seed) {
+  XPFC_ROUTE(c, s, xpfc_inst_initialize_noise_noalloc(cc, ss, amplitude, seed));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!std::isfinite(amplitude)) return fail(XPFC_ERR_ARG, "non-finite amplitude");
@@ -1019,6 +1091,30 @@ int drain_report(xpfc_ctx* c) {
   return XPFC_OK;
 }
 
+int fe_now(xpfc_ctx* c, int s, double* F, double* fm, double* mean_rs);
+
+// main.cu:286: the FE record of the initial field of every active instance
+// that samples and has not stepped yet
+int initial_records(xpfc_ctx* c) {
+  for (int s : c->active) {
+    XInst& in = c->inst[s];
+    if (in.step == 0 && in.p.t_sample > 0 && in.samples.empty()) {
+      double F, fm, mrs;
+      int rc = fe_now(c, s, &F, &fm, &mrs);
+      if (rc) return rc;
+      xpfc_sample r{};
+      r.step = 0;
+      r.has_fe = 1;
+      r.F_ref = F;
+      r.f_mean = fm;
+      r.mean = mrs;
+      r.rmse = NAN;
+      in.samples.push_back(r);
+    }
+  }
+  return XPFC_OK;
+}
+
 int fe_now(xpfc_ctx* c, int s, double* F, double* fm, double* mean_rs) {
   int nb = 0;
   const Batch bt = bt_one(c, s);
@@ -1195,6 +1291,10 @@ bool autotune_enabled() {
 
 extern "C" int xpfc_autotune(xpfc_ctx* c, int* variants) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!c->cls.empty()) {  // one launch shape per pass for every grid: nothing to choose
+    if (variants) variants[0] = variants[1] = variants[2] = 0;
+    return XPFC_OK;
+  }
   int rc = autotune(c);
   if (rc) return rc;
   if (variants) {
@@ -1207,6 +1307,7 @@ extern "C" int xpfc_autotune(xpfc_ctx* c, int* variants) {
 
 extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
   if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  XPFC_NOT_MIXED(c, "xpfc_set_variants");
   const long long nl = (long long)c->n0 * c->n1;
   if (v[0] > 0 && strided_blocks(c->prec, c->n0, v[0], c->segX) < 0)
     return fail(XPFC_ERR_ARG, "tile width %d not available for n0 = %d", v[0], c->n0);
@@ -1224,6 +1325,10 @@ extern "C" int xpfc_set_variants(xpfc_ctx* c, const int* v) {
 
 extern "C" int xpfc_get_variants(const xpfc_ctx* c, int* v) {
   if (!c || !v) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->cls.empty()) {
+    v[0] = v[1] = v[2] = 0;
+    return XPFC_OK;
+  }
   v[0] = c->tc_x;
   v[1] = c->tc_y;
   v[2] = c->ntz;
@@ -1233,13 +1338,18 @@ extern "C" int xpfc_get_variants(const xpfc_ctx* c, int* v) {
 extern "C" int xpfc_set_graphs(xpfc_ctx* c, int on) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   c->use_graphs = on != 0;
-  if (!c->use_graphs) drop_graphs(c);
+  if (!c->use_graphs) {
+    drop_graphs(c);
+    for (auto& g : c->rgraphs) (void)hipGraphExecDestroy(g.second);
+    c->rgraphs.clear();
+  }
   return XPFC_OK;
 }
 
 extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (nsteps < 0) return fail(XPFC_ERR_ARG, "nsteps < 0");
+  if (!c->cls.empty()) return mixed_step(c, nsteps);
   if (c->active.empty() || nsteps == 0) return XPFC_OK;
   for (int s : c->active)
     if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "instance %d: no field set", s);
@@ -1247,23 +1357,7 @@ extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
     int rc = autotune(c);
     if (rc) return rc;
   }
-  for (int s : c->active) {
-    XInst& in = c->inst[s];
-    if (in.step == 0 && in.p.t_sample > 0 && in.samples.empty()) {
-      // main.cu:286: FE of the initial field
-      double F, fm, mrs;
-      int rc = fe_now(c, s, &F, &fm, &mrs);
-      if (rc) return rc;
-      xpfc_sample r{};
-      r.step = 0;
-      r.has_fe = 1;
-      r.F_ref = F;
-      r.f_mean = fm;
-      r.mean = mrs;
-      r.rmse = NAN;
-      in.samples.push_back(r);
-    }
-  }
+  if (int rc = initial_records(c)) return rc;
   auto any_samples = [&](long long off) {
     for (int s : c->active)
       if (inst_samples(c->inst[s], c->inst[s].step + off)) return true;
@@ -1298,7 +1392,9 @@ extern "C" int xpfc_step(xpfc_ctx* c, int nsteps) {
 }
 
 extern "C" long long xpfc_inst_current_step(const xpfc_ctx* c, int s) {
-  return (c && s >= 0 && s < c->K) ? c->inst[s].step : 0;
+  if (!c || s < 0 || s >= c->K) return 0;
+  if (!c->cls.empty()) return xpfc_inst_current_step(c->cls[c->where[s].first], c->where[s].second);
+  return c->inst[s].step;
 }
 extern "C" long long xpfc_current_step(const xpfc_ctx* c) { return xpfc_inst_current_step(c, 0); }
 
@@ -1315,10 +1411,11 @@ extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  return drain_report(c);
+  return c->cls.empty() ? drain_report(c) : mixed_sync(c);
 }
 
 extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
+  XPFC_NOT_MIXED(c, "xpfc_time_passes");
   if (!c || !ms || nsteps <= 0) return fail(XPFC_ERR_ARG, "bad arguments");
   for (int s : c->active)
     if (!c->inst[s].have_field) return fail(XPFC_ERR_STATE, "no field set");
@@ -1360,6 +1457,7 @@ extern "C" int xpfc_query(xpfc_ctx* c) {
 
 extern "C" int xpfc_probe_finite(xpfc_ctx* c, int* finite) {
   if (!c || !finite) return fail(XPFC_ERR_ARG, "null argument");
+  if (!c->cls.empty()) return mixed_probe(c, finite);
   LCHK(launch_probe(c->prec, c->psi, c->stride, c->nyoff, c->K, c->d_flags, c->stream));
   HIPCHK(hipMemcpyAsync(finite, c->d_flags, (size_t)c->K * sizeof(int), hipMemcpyDeviceToHost,
                         c->stream));
@@ -1369,18 +1467,21 @@ extern "C" int xpfc_probe_finite(xpfc_ctx* c, int* finite) {
 
 extern "C" int xpfc_sync(xpfc_ctx* c) {
   if (!c) return fail(XPFC_ERR_ARG, "null context");
+  if (!c->cls.empty()) return mixed_sync(c);
   HIPCHK(hipStreamSynchronize(c->stream));
   return drain_report(c);
 }
 
 // counts on success, minus the status code on failure
 extern "C" int xpfc_inst_num_samples(xpfc_ctx* c, int s) {
+  XPFC_ROUTE(c, s, xpfc_inst_num_samples(cc, ss));
   if (!c || s < 0 || s >= c->K) return -fail(XPFC_ERR_ARG, "bad arguments");
   int rc = drain(c);
   if (rc) return -rc;
   return (int)c->inst[s].samples.size();
 }
 extern "C" int xpfc_inst_get_samples(xpfc_ctx* c, int s, xpfc_sample* out, int max) {
+  XPFC_ROUTE(c, s, xpfc_inst_get_samples(cc, ss, out, max));
   if (!c || s < 0 || s >= c->K || (max > 0 && !out)) return -fail(XPFC_ERR_ARG, "bad arguments");
   int rc = drain(c);
   if (rc) return -rc;
@@ -1390,6 +1491,10 @@ extern "C" int xpfc_inst_get_samples(xpfc_ctx* c, int s, xpfc_sample* out, int m
   return n;
 }
 extern "C" void xpfc_inst_clear_samples(xpfc_ctx* c, int s) {
+  if (c && !c->cls.empty() && s >= 0 && s < c->K) {
+    xpfc_inst_clear_samples(c->cls[c->where[s].first], c->where[s].second);
+    return;
+  }
   if (c && s >= 0 && s < c->K && drain(c) == XPFC_OK) {
     c->inst[s].samples.clear();
     c->inst[s].nonfinite = c->inst[s].nonfinite_reported = false;
@@ -1405,6 +1510,7 @@ extern "C" void xpfc_clear_samples(xpfc_ctx* c) { xpfc_inst_clear_samples(c, 0);
 // diagnostics
 // ---------------------------------------------------------------------------
 extern "C" int xpfc_inst_fe_calc(xpfc_ctx* c, int s, double* F_ref, double* f_mean) {
+  XPFC_ROUTE(c, s, xpfc_inst_fe_calc(cc, ss, F_ref, f_mean));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!F_ref || !f_mean) return fail(XPFC_ERR_ARG, "null argument");
@@ -1414,6 +1520,7 @@ extern "C" int xpfc_inst_fe_calc(xpfc_ctx* c, int s, double* F_ref, double* f_me
 }
 
 extern "C" int xpfc_inst_avg_calc(xpfc_ctx* c, int s, double* mean) {
+  XPFC_ROUTE(c, s, xpfc_inst_avg_calc(cc, ss, mean));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!mean) return fail(XPFC_ERR_ARG, "null argument");
@@ -1423,6 +1530,7 @@ extern "C" int xpfc_inst_avg_calc(xpfc_ctx* c, int s, double* mean) {
 }
 
 extern "C" int xpfc_inst_sse_calc(xpfc_ctx* c, int s, double* rmse) {
+  XPFC_ROUTE(c, s, xpfc_inst_sse_calc(cc, ss, rmse));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!rmse) return fail(XPFC_ERR_ARG, "null argument");
@@ -1474,6 +1582,7 @@ const Phys& ph0(const xpfc_ctx* c) { return c->inst[0].ia.ph; }
 }  // namespace
 
 extern "C" int xpfc_power_functions(xpfc_ctx* c, int p, const void* src, void* dst) {
+  XPFC_NOT_MIXED(c, "xpfc_power_functions");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!src || !dst || p < 0) return fail(XPFC_ERR_ARG, "bad power_functions arguments");
   LCHK(launch_power(c->prec, src, dst, c->N, p, c->inst[0].p.powf_fp32 != 0, c->stream));
@@ -1481,6 +1590,7 @@ extern "C" int xpfc_power_functions(xpfc_ctx* c, int p, const void* src, void* d
 }
 
 extern "C" int xpfc_convolve_fields(xpfc_ctx* c, const void* psi_k, void* out) {
+  XPFC_NOT_MIXED(c, "xpfc_convolve_fields");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!psi_k || !out) return fail(XPFC_ERR_ARG, "null buffer");
   LCHK(launch_convolve(c->prec, psi_k, out, c->g, ph0(c), c->n2, c->stream));
@@ -1489,6 +1599,7 @@ extern "C" int xpfc_convolve_fields(xpfc_ctx* c, const void* psi_k, void* out) {
 
 extern "C" int xpfc_euler_step(xpfc_ctx* c, const void* psi_k, const void* p2_k, const void* p3_k,
                                void* out) {
+  XPFC_NOT_MIXED(c, "xpfc_euler_step");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!psi_k || !p2_k || !p3_k || !out) return fail(XPFC_ERR_ARG, "null buffer");
   const xpfc_params& p = c->inst[0].p;
@@ -1498,6 +1609,7 @@ extern "C" int xpfc_euler_step(xpfc_ctx* c, const void* psi_k, const void* p2_k,
 
 extern "C" int xpfc_residual_calc(xpfc_ctx* c, const void* kp1, const void* k0, const void* p2,
                                   const void* p3, void* out) {
+  XPFC_NOT_MIXED(c, "xpfc_residual_calc");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!kp1 || !k0 || !p2 || !p3 || !out) return fail(XPFC_ERR_ARG, "null buffer");
   const xpfc_params& p = c->inst[0].p;
@@ -1506,6 +1618,7 @@ extern "C" int xpfc_residual_calc(xpfc_ctx* c, const void* kp1, const void* k0, 
 }
 
 extern "C" int xpfc_scaling_step(xpfc_ctx* c, void* buf, int spectral) {
+  XPFC_NOT_MIXED(c, "xpfc_scaling_step");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!buf) return fail(XPFC_ERR_ARG, "null buffer");
   const long long n = spectral ? 2LL * c->n0 * c->n1 * (c->H + 1) : c->N;
@@ -1526,6 +1639,7 @@ int scratch_args(xpfc_ctx* c, double mean, double scale) {
 }  // namespace
 
 extern "C" int xpfc_fft_r2c(xpfc_ctx* c, const void* real_in, void* spec_out) {
+  XPFC_NOT_MIXED(c, "xpfc_fft_r2c");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!real_in || !spec_out) return fail(XPFC_ERR_ARG, "null buffer");
   const long long nl = (long long)c->n0 * c->n1;
@@ -1542,6 +1656,7 @@ extern "C" int xpfc_fft_r2c(xpfc_ctx* c, const void* real_in, void* spec_out) {
 }
 
 extern "C" int xpfc_fft_c2r(xpfc_ctx* c, const void* spec_in, void* real_out) {
+  XPFC_NOT_MIXED(c, "xpfc_fft_c2r");
   if (!c) return fail(XPFC_ERR_ARG, "null context");
   if (!spec_in || !real_out) return fail(XPFC_ERR_ARG, "null buffer");
   const long long nl = (long long)c->n0 * c->n1;
@@ -1560,6 +1675,7 @@ extern "C" int xpfc_fft_c2r(xpfc_ctx* c, const void* spec_in, void* real_out) {
 // VTK output (write_vtk.cu:4-35)
 // ---------------------------------------------------------------------------
 extern "C" int xpfc_inst_write_vtk(xpfc_ctx* c, int s, const char* path) {
+  XPFC_ROUTE(c, s, xpfc_inst_write_vtk(cc, ss, path));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!path) return fail(XPFC_ERR_ARG, "null path");
@@ -1610,6 +1726,7 @@ const char kMagic[8] = {'X', 'P', 'F', 'C', 'C', 'K', 'P', '1'};
 }  // namespace
 
 extern "C" int xpfc_inst_save_checkpoint(xpfc_ctx* c, int s, const char* path) {
+  XPFC_ROUTE(c, s, xpfc_inst_save_checkpoint(cc, ss, path));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!path) return fail(XPFC_ERR_ARG, "null path");
@@ -1641,6 +1758,7 @@ extern "C" int xpfc_inst_save_checkpoint(xpfc_ctx* c, int s, const char* path) {
 }
 
 extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
+  XPFC_ROUTE(c, s, xpfc_inst_load_checkpoint(cc, ss, path));
   int rc = check_inst(c, s);
   if (rc) return rc;
   if (!path) return fail(XPFC_ERR_ARG, "null path");
@@ -1688,4 +1806,381 @@ extern "C" int xpfc_save_checkpoint(xpfc_ctx* c, const char* path) {
 }
 extern "C" int xpfc_load_checkpoint(xpfc_ctx* c, const char* path) {
   return xpfc_inst_load_checkpoint(c, 0, path);
+}
+
+// ---------------------------------------------------------------------------
+// contexts of several grids (xpfc_create_mixed): the instances of each grid
+// form a class context; all classes share one stream and advance together
+// with one ragged launch per pass (every class's tiles in one grid of
+// workgroups), captured in hipGraph step blocks like the one-grid path
+// ---------------------------------------------------------------------------
+namespace {
+
+// (dim, extents, box) of a parameter set, 2-D normalised
+std::vector<double> grid_key(const xpfc_params& p0) {
+  xpfc_params p = p0;
+  if (p.dim == 2) {
+    p.n[2] = 1;
+    p.L[2] = 1.0;
+  }
+  return {(double)p.dim, (double)p.n[0], (double)p.n[1], (double)p.n[2], p.L[0], p.L[1], p.L[2],
+          (double)p.precision, (double)p.device};
+}
+
+std::vector<int> active_counts(const xpfc_ctx* c) {
+  std::vector<int> k;
+  for (const xpfc_ctx* q : c->cls) k.push_back((int)q->active.size());
+  return k;
+}
+
+// segments with the tile counts of tile width tc (as the launchers fill them)
+Segs filled(const Segs& in, int tc) {
+  Segs s = in;
+  int tiles[2];
+  for (int i = 0; i < 2; ++i) {
+    s.s[i].ntc = (s.s[i].ncols + tc - 1) / tc;
+    tiles[i] = (s.s[i].ncols <= 0 || s.s[i].nbatch <= 0) ? 0 : s.s[i].ntc * s.s[i].nbatch;
+  }
+  s.tiles0 = tiles[0];
+  s.tiles1 = tiles[1];
+  s.lgc = -1;
+  return s;
+}
+
+// Workgroup order of a class's strided-pass segment of n workgroups (tiles of
+// all its instances, instance-major): runs of up to 2^XCD_RUN_X neighbouring
+// tiles per XCD, at least two rounds of them (as the one-grid axis-0 pass).
+int rag_lgc(int n) {
+  int lg = XCD_RUN_X;
+  while (lg >= 0 && n < (16 << lg)) --lg;
+  return lg >= 2 ? lg : -1;
+}
+int round8(int n) { return (n + 7) & ~7; }
+
+// The class tables of the current active lists (built once per distinct
+// list lengths; the kernels read the lists themselves from device memory).
+int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
+  std::vector<int> key = active_counts(c);
+  key.push_back(budget ? 1 : 0);
+  auto it = c->rtab.find(key);
+  if (it != c->rtab.end()) {
+    *out = &it->second;
+    return XPFC_OK;
+  }
+  xpfc_ctx::RagTables t;
+  std::vector<RClass> X, XB, Y, Z;  // axis 0 with C2 tables (square / cubic boxes), axis 0 otherwise
+  const int nc = (int)c->cls.size();
+  t.nbx.assign(nc, 0);
+  t.nbz.assign(nc, 0);
+  for (int k = 0; k < nc; ++k) {
+    xpfc_ctx* q = c->cls[k];
+    const int na = (int)q->active.size();
+    t.nbx[k] = rag_strided_wgs(q->prec, q->n0, q->segX);
+    t.nbz[k] = rag_zpass_wgs(q->prec, q->H, (long long)q->n0 * q->n1);
+    const int nby = q->n1 > 1 ? rag_strided_wgs(q->prec, q->n1, q->segY) : 0;
+    if (t.nbx[k] <= 0 || t.nbz[k] <= 0 || nby < 0 || q->H > 512)
+      return fail(XPFC_ERR_ARG, "grid %d x %d x %d has no one-launch form", q->n0, q->n1, q->n2);
+    if (na == 0) continue;
+    const Batch bt = bt_active(q);
+    int& wx = q->g.cubic ? t.wx : t.wxb;
+    RClass r{};
+    r.L = q->n0;
+    r.wg0 = wx = round8(wx);
+    r.lgc = rag_lgc(na * t.nbx[k]);
+    r.wgs = t.nbx[k];
+    r.nact = na;
+    r.cubic = q->g.cubic;
+    r.segs = filled(q->segX, rag_tile(q->n0));
+    r.g = q->g;
+    r.bt = bt;
+    r.a = q->work;
+    r.b = q->psi;
+    r.tw = q->tw0;
+    r.partial = q->px;
+    r.budget = budget ? q->d_budget : nullptr;
+    (q->g.cubic ? X : XB).push_back(r);
+    wx += na * r.wgs;
+    t.lx = std::max(t.lx, rag_strided_lds(q->prec, q->n0));
+    if (q->n1 > 1) {
+      RClass y{};
+      y.L = q->n1;
+      y.wg0 = t.wy = round8(t.wy);
+      y.lgc = -1;
+      y.wgs = nby;
+      y.nact = na;
+      y.segs = filled(q->segY, rag_tile(q->n1));
+      y.bt = bt;
+      y.a = q->work;
+      y.tw = q->tw1;
+      y.budget = r.budget;
+      Y.push_back(y);
+      t.wy += na * nby;
+      t.ly = std::max(t.ly, rag_strided_lds(q->prec, q->n1));
+    }
+    RClass z{};
+    z.L = q->H;
+    z.wg0 = t.wz;
+    z.wgs = t.nbz[k];
+    z.nact = na;
+    z.pw = q->any_powf;
+    z.bt = bt;
+    z.a = q->work;
+    z.tw = q->twz;
+    z.twp = q->twp;
+    z.partial = q->pz;
+    z.nlines = (long long)q->n0 * q->n1;
+    z.zl = ZLines{q->n1, q->xs, q->xs};
+    z.nyoff = q->nyoff;
+    z.budget = r.budget;
+    Z.push_back(z);
+    t.wz += na * z.wgs;
+    t.lz = std::max(t.lz, rag_zpass_lds(q->prec, q->H));
+  }
+  t.nx = (int)X.size();
+  t.nxb = (int)XB.size();
+  t.ny = (int)Y.size();
+  t.nz = (int)Z.size();
+  if (t.nz > RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids active at once", RAG_MAX);
+  LCHK(rag_prepare(c->prec, t.lx, t.ly, t.lz));
+  std::vector<RClass> all(X);
+  all.insert(all.end(), XB.begin(), XB.end());
+  all.insert(all.end(), Y.begin(), Y.end());
+  all.insert(all.end(), Z.begin(), Z.end());
+  if (!all.empty()) {
+    if (hipMalloc((void**)&t.d, all.size() * sizeof(RClass)) != hipSuccess)
+      return fail(XPFC_ERR_NOMEM, "class table allocation failed");
+    if (hipMemcpy(t.d, all.data(), all.size() * sizeof(RClass), hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipFree(t.d);
+      return fail(XPFC_ERR_DEVICE, "class table upload failed");
+    }
+  }
+  *out = &(c->rtab[key] = t);
+  return XPFC_OK;
+}
+
+bool mixed_pw(const xpfc_ctx* c) {
+  for (const xpfc_ctx* q : c->cls)
+    if (!q->active.empty() && q->any_powf) return true;
+  return false;
+}
+
+// The launches of one step of every active instance of every class: axis 0
+// (a second launch only when square / cubic and other boxes are mixed), axis
+// 1 inverse (3-D classes), real space, axis 1 forward (3-D classes),
+// then for sampled steps each listed class's finalize into its row `rows[k]`.
+int mixed_one_step(xpfc_ctx* c, const xpfc_ctx::RagTables& t, bool sample, int sib,
+                   const std::vector<int>* rows, bool pw) {
+  const RClass* X = t.d;
+  const RClass* XB = X + t.nx;
+  const RClass* Y = XB + t.nxb;
+  const RClass* Z = Y + t.ny;
+  if (t.nx) LCHK(launch_xstep_ragged(c->prec, X, t.nx, t.wx, t.lx, true, sample, sib, c->stream));
+  if (t.nxb) LCHK(launch_xstep_ragged(c->prec, XB, t.nxb, t.wxb, t.lx, false, sample, sib, c->stream));
+  if (t.ny) LCHK(launch_strided_ragged(c->prec, Y, t.ny, t.wy, t.ly, +1, sib, c->stream));
+  LCHK(launch_zpass_ragged(c->prec, Z, t.nz, t.wz, t.lz, sample, pw, sib, c->stream));
+  if (t.ny) LCHK(launch_strided_ragged(c->prec, Y, t.ny, t.wy, t.ly, -1, sib, c->stream));
+  if (sample && rows) {
+    for (size_t k = 0; k < c->cls.size(); ++k) {
+      if ((*rows)[k] < 0) continue;
+      xpfc_ctx* q = c->cls[k];
+      LCHK(launch_finalize(q->px, t.nbx[k], q->pz, t.nbz[k], bt_active(q), (int)q->active.size(),
+                           q->sums + (size_t)(*rows)[k] * q->K * NSUMS, c->stream));
+    }
+  }
+  return XPFC_OK;
+}
+
+void mixed_drop_graphs(xpfc_ctx* c) {
+  for (auto& g : c->rgraphs) (void)hipGraphExecDestroy(g.second);
+  c->rgraphs.clear();
+}
+
+// Unsampled steps: replays of captured blocks of GRAPH_BLOCK steps (or a
+// power of two below), keyed by length, active counts and the power form.
+int mixed_plain(xpfc_ctx* c, const xpfc_ctx::RagTables& t, long long run, bool pw) {
+  while (run > 0) {
+    if (!c->use_graphs) {
+      int rc = mixed_one_step(c, t, false, 0, nullptr, pw);
+      if (rc) return rc;
+      for (xpfc_ctx* q : c->cls) advance(q, 1);
+      --run;
+      continue;
+    }
+    int len = GRAPH_BLOCK;
+    if (run < GRAPH_BLOCK) {
+      len = 1;
+      while (2 * len <= run) len *= 2;
+    }
+    std::vector<int> key = active_counts(c);
+    key.push_back(len);
+    key.push_back(pw ? 1 : 0);
+    hipGraphExec_t ex = nullptr;
+    auto it = c->rgraphs.find(key);
+    if (it != c->rgraphs.end()) {
+      ex = it->second;
+    } else {
+      if (c->rgraphs.size() >= 64) mixed_drop_graphs(c);
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = XPFC_OK;
+      for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = mixed_one_step(c, t, false, i, nullptr, pw);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rc != XPFC_OK || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != XPFC_OK ? rc : fail(XPFC_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(ec));
+      }
+      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+      c->rgraphs[key] = ex;
+    }
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+    for (xpfc_ctx* q : c->cls) advance(q, len);
+    run -= len;
+  }
+  return XPFC_OK;
+}
+
+int mixed_step(xpfc_ctx* c, int nsteps) {
+  if (nsteps == 0) return XPFC_OK;
+  bool any = false;
+  for (xpfc_ctx* q : c->cls) {
+    for (int s : q->active)
+      if (!q->inst[s].have_field) return fail(XPFC_ERR_STATE, "an instance of grid %d x %d x %d has no field",
+                                              q->n0, q->n1, q->n2);
+    any = any || !q->active.empty();
+  }
+  if (!any) return XPFC_OK;
+  for (xpfc_ctx* q : c->cls)
+    if (int rc = initial_records(q)) return rc;
+  const xpfc_ctx::RagTables* t = nullptr;
+  if (int rc = mixed_tables(c, false, &t)) return rc;
+  const bool pw = mixed_pw(c);
+  auto samples_at = [&](long long off) {
+    for (const xpfc_ctx* q : c->cls)
+      for (int s : q->active)
+        if (inst_samples(q->inst[s], q->inst[s].step + off)) return true;
+    return false;
+  };
+  long long left = nsteps;
+  while (left > 0) {
+    if (samples_at(0)) {
+      std::vector<int> rows(c->cls.size(), -1);
+      for (size_t k = 0; k < c->cls.size(); ++k) {
+        xpfc_ctx* q = c->cls[k];
+        XPending pr;
+        for (int s : q->active)
+          if (inst_samples(q->inst[s], q->inst[s].step)) pr.who.emplace_back(s, q->inst[s].step);
+        if (pr.who.empty()) continue;
+        if ((int)q->pending.size() == SUM_ROWS)
+          if (int rc = drain(q)) return rc;
+        pr.row = (int)q->pending.size();
+        rows[k] = pr.row;
+        q->pending.push_back(pr);
+      }
+      if (int rc = mixed_one_step(c, *t, true, 0, &rows, pw)) return rc;
+      for (xpfc_ctx* q : c->cls) advance(q, 1);
+      --left;
+      continue;
+    }
+    long long run = 1;
+    while (run < left && !samples_at(run)) ++run;
+    if (int rc = mixed_plain(c, *t, run, pw)) return rc;
+    left -= run;
+  }
+  return XPFC_OK;
+}
+
+int mixed_set_active(xpfc_ctx* c, const int* slots, int n) {
+  std::vector<std::vector<int>> per(c->cls.size());
+  if (!slots) {
+    for (int s = 0; s < c->K; ++s) per[c->where[s].first].push_back(c->where[s].second);
+  } else {
+    if (n < 0 || n > c->K) return fail(XPFC_ERR_ARG, "bad active count");
+    std::vector<char> seen(c->K, 0);
+    for (int i = 0; i < n; ++i) {
+      if (slots[i] < 0 || slots[i] >= c->K || seen[slots[i]])
+        return fail(XPFC_ERR_ARG, "bad or repeated instance %d", slots[i]);
+      seen[slots[i]] = 1;
+      per[c->where[slots[i]].first].push_back(c->where[slots[i]].second);
+    }
+  }
+  const int none = 0;
+  for (size_t k = 0; k < c->cls.size(); ++k)
+    if (int rc = xpfc_set_active(c->cls[k], per[k].empty() ? &none : per[k].data(), (int)per[k].size()))
+      return rc;
+  return XPFC_OK;
+}
+
+int mixed_sync(xpfc_ctx* c) {
+  HIPCHK(hipStreamSynchronize(c->stream));
+  int worst = XPFC_OK;
+  for (xpfc_ctx* q : c->cls) {
+    const int rc = drain_report(q);
+    if (rc == XPFC_ERR_NONFINITE) worst = rc;  // keep reading the other classes
+    else if (rc) return rc;
+  }
+  return worst;
+}
+
+int mixed_probe(xpfc_ctx* c, int* finite) {
+  for (size_t k = 0; k < c->cls.size(); ++k) {
+    xpfc_ctx* q = c->cls[k];
+    std::vector<int> f((size_t)q->K);
+    if (int rc = xpfc_probe_finite(q, f.data())) return rc;
+    for (int s = 0; s < c->K; ++s)
+      if (c->where[s].first == (int)k) finite[s] = f[(size_t)c->where[s].second];
+  }
+  return XPFC_OK;
+}
+
+}  // namespace
+
+extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
+  if (!ps || !out || K < 1) return fail(XPFC_ERR_ARG, "bad arguments");
+  *out = nullptr;
+  for (int s = 1; s < K; ++s)
+    if (ps[s].precision != ps[0].precision || ps[s].device != ps[0].device)
+      return fail(XPFC_ERR_ARG, "instance %d: precision or device differ from instance 0", s);
+  // classes in order of first appearance
+  std::vector<std::vector<double>> keys;
+  std::vector<std::vector<int>> members;
+  for (int s = 0; s < K; ++s) {
+    const std::vector<double> k = grid_key(ps[s]);
+    size_t i = 0;
+    while (i < keys.size() && keys[i] != k) ++i;
+    if (i == keys.size()) {
+      keys.push_back(k);
+      members.emplace_back();
+    }
+    members[i].push_back(s);
+  }
+  if ((int)keys.size() > RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids in one context", RAG_MAX);
+  xpfc_ctx* c = new (std::nothrow) xpfc_ctx();
+  if (!c) return fail(XPFC_ERR_NOMEM, "host allocation failed");
+  auto bail = [&](int r) {
+    free_all(c);
+    delete c;
+    return r;
+  };
+  c->p = ps[0];
+  c->K = K;
+  c->prec = ps[0].precision == XPFC_FP32_MIXED ? 1 : 0;
+  if (hipSetDevice(ps[0].device) != hipSuccess)
+    return bail(fail(XPFC_ERR_DEVICE, "hipSetDevice(%d) failed", ps[0].device));
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
+  c->where.assign(K, {0, 0});
+  for (size_t k = 0; k < members.size(); ++k) {
+    std::vector<xpfc_params> gp;
+    for (int s : members[k]) gp.push_back(ps[s]);
+    xpfc_ctx* q = nullptr;
+    if (int rc = create_impl(gp.data(), (int)gp.size(), c->stream, &q)) return bail(rc);
+    c->cls.push_back(q);
+    for (size_t i = 0; i < members[k].size(); ++i) c->where[members[k][i]] = {(int)k, (int)i};
+  }
+  const char* eg = std::getenv("XPFC_GRAPHS");
+  c->use_graphs = !(eg && eg[0] == '0');
+  *out = c;
+  return XPFC_OK;
 }

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <utility>
 #include <vector>
@@ -104,4 +105,21 @@ struct xpfc_ctx {
   std::vector<XPending> pending;
   std::vector<double> hsums;         // host rows for drain: [SUM_ROWS][K][NSUMS]
   bool nonfinite_unreported = false;  // an instance diverged since the last report
+  int* d_budget = nullptr;           // [K] steps of the current block per slot (mixed contexts)
+
+  // ---- mixed-grid contexts (xpfc_create_mixed): instances of several grids.
+  // Each grid is a class context holding that grid's instances (sharing this
+  // context's stream); instance s is slot where[s].second of class where[s].first.
+  std::vector<xpfc_ctx*> cls;
+  std::vector<std::pair<int, int>> where;
+  bool borrowed_stream = false;      // a class context: the stream belongs to its owner
+  struct RagTables {                 // device class tables of one active-count configuration
+    xpfc::RClass* d = nullptr;       // x (square / cubic boxes) | x (others) | y | z, back to back
+    int nx = 0, nxb = 0, ny = 0, nz = 0;  // classes in each
+    int wx = 0, wxb = 0, wy = 0, wz = 0;  // workgroups of each launch
+    size_t lx = 0, ly = 0, lz = 0;   // dynamic LDS bytes of each kind of launch
+    std::vector<int> nbx, nbz;       // per class: X / Z partial sums per instance
+  };
+  std::map<std::vector<int>, RagTables> rtab;                  // key: active counts (+ budget flag)
+  std::map<std::vector<int>, hipGraphExec_t> rgraphs;          // key: length, counts, flags
 };

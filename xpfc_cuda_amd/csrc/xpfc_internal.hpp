@@ -105,6 +105,44 @@ constexpr int XPART = 2;
 constexpr int ZPART = 4;
 constexpr int NSUMS = 6;
 
+// ---------------------------------------------------------------------------
+// Ragged launches: one launch per pass over the instances of several grids
+// (size classes).  Workgroups [wg0, wg0 + nact * wgs) of the launch belong to
+// class k, instance-major; a workgroup finds its class by scanning wg0.  The
+// workgroup size is fixed per pass (RAG_NT for the strided passes, RAG_NTZ
+// for the real-space pass), so a class with lines of L points takes tiles of
+// RAG_NT * 8 / L columns (RAG_NTZ / (threads per line) lines); the arithmetic
+// of every line is that of the one-grid kernels.
+// ---------------------------------------------------------------------------
+constexpr int RAG_MAX = 16;    // classes per ragged launch
+constexpr int RAG_NT = 512;    // threads per workgroup, strided passes
+constexpr int RAG_NTZ = 256;   // threads per workgroup, real-space pass
+struct RClass {
+  int L;            // line length of the pass (n0, n1, or H = n2 / 2)
+  int wg0;          // first workgroup of the class
+  int wgs;          // workgroups per instance
+  int nact;         // instances of the class in this launch
+  int cubic;        // axis-0 pass: C2 from the table over m
+  int pw;           // real-space pass: some instance forms the FP32 powers
+  int lgc;          // strided passes: workgroup order over the XCDs within the class
+                    // (runs of 2^lgc tiles, or < 0: one run per XCD); wg0 is a
+                    // multiple of 8, so a workgroup's XCD is its index in the class mod 8
+  Segs segs;        // strided passes: the class's segments, tiles filled
+  KGeom g;          // axis-0 pass
+  Batch bt;         // slots (the class's active list), instance table, strides
+  void* a;          // axis-0: work; axis-1 and real space: the buffer (in place)
+  void* b;          // axis-0: psi
+  const void* tw;   // line twiddles
+  const void* twp;  // real space: half-spectrum twiddles
+  double* partial;  // sampled partial sums (X or Z) of the class
+  long long nlines; // real space: lines per instance
+  ZLines zl;        // real space
+  long long nyoff;  // real space: offset of the Nyquist plane (complex elements)
+  const int* budget;  // steps of the current block per slot (nullptr: every step)
+};
+// Column tile width of a ragged strided pass over lines of L points.
+constexpr int rag_tile(int L) { return L >= 8 ? RAG_NT * 8 / L : 0; }
+
 // Launch entry points of the step passes (xpfc_passes.hip).  `tc` is the tile
 // width (4, 8 or 16 columns per workgroup) of a strided pass and `ntz` the
 // workgroup size (64..512) of the real-space pass; a combination a size does
@@ -128,6 +166,25 @@ int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* ou
                  const void* tw, const void* twp, double* partial, bool sample, bool pw,
                  hipStream_t s);
 int zpass_blocks(int prec, int H, int ntz, long long nlines);
+// Ragged launches over the device table `cls` (ncls classes, `nwg` workgroups
+// in all, `lds` bytes of dynamic LDS).  `sib`: index of the step within its
+// block, compared with the per-slot budgets.  Return 0, or -2 / -3 as above.
+// (`lut`: every class of this table has a square / cubic box)
+int launch_xstep_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool lut, bool sample,
+                        int sib, hipStream_t s);
+int launch_strided_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib,
+                          hipStream_t s);
+int launch_zpass_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw,
+                        int sib, hipStream_t s);
+// Workgroups per instance and dynamic LDS bytes of a class in a ragged launch
+// (0 / negative: lines of that length are not supported).
+// Lets the ragged kernels of precision prec use the given dynamic LDS (call
+// before launching or capturing them).
+int rag_prepare(int prec, size_t lx, size_t ly, size_t lz);
+int rag_strided_wgs(int prec, int L, const Segs& segs);
+size_t rag_strided_lds(int prec, int L);
+int rag_zpass_wgs(int prec, int H, long long nlines);
+size_t rag_zpass_lds(int prec, int H);
 int strided_blocks(int prec, int L, int tc, const Segs& segs);
 // out[slot * NSUMS + k]: the six sums of every listed instance
 int launch_finalize(const double* px, int nx, const double* pz, int nz, const Batch& bt, int ny,

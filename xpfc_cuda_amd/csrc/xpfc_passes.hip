@@ -93,16 +93,14 @@ __device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, in
 // real-space pass after the inverse one -6 %, the forward axis-1 pass itself
 // +7 %; the step -2.3 % in FP32, -4 % in FP64); for fields that fit the cache
 // the hint costs up to 15 %.
+// One column tile of a strided pass (buf already at the instance's spectra).
 template <typename T, int L, int TC, int DIR, bool NT>
-__global__ __launch_bounds__((tile_threads<L, TC>()))
-void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
+__device__ __forceinline__ void strided_body(cpx<T>* __restrict__ buf, const Segs& segs, int tile,
+                                             const cpx<T>* __restrict__ tw, cpx<T>* lds, cpx<T>* twl) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
-  __shared__ C lds[L * (TC + PADT)];
-  __shared__ C twl[L];
-  buf += (long long)bt.slots[blockIdx.y] * bt.stride;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
-  const TileLoc loc = locate(segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ci, TC);
+  const TileLoc loc = locate(segs, tile, ci, TC);
   const int rs = loc.rs;
   C* p = buf + loc.off;
   C v[E];
@@ -121,6 +119,16 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
       else *q = v[m];
     }
   }
+}
+
+template <typename T, int L, int TC, int DIR, bool NT>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
+void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
+  using C = cpx<T>;
+  __shared__ C lds[L * (TC + PADT)];
+  __shared__ C twl[L];
+  buf += (long long)bt.slots[blockIdx.y] * bt.stride;
+  strided_body<T, L, TC, DIR, NT>(buf, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), tw, lds, twl);
 }
 
 // ---------------------------------------------------------------------------
@@ -255,30 +263,40 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   }
 }
 
+// One column tile of the fused axis-0 pass (work, psi, partial already at the
+// instance's regions; `wb`: the workgroup's partial-sum slot).
 template <typename T, int L, int TC, bool SAMPLE, bool LUT>
-__global__ __launch_bounds__((tile_threads<L, TC>()))
-void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
-             const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
-  using C = cpx<T>;
+__device__ __forceinline__ void xstep_body(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi,
+                                           const Segs& segs, int tile, const cpx<T>* __restrict__ tw,
+                                           const KGeom& g, const Phys& ph, cpx<T>* lds, cpx<T>* twl,
+                                           double* __restrict__ partial, int wb) {
   constexpr int E = elems_per_thread<L>();
   constexpr int NT = tile_threads<L, TC>();
-  __shared__ C lds[L * (TC + PADT)];
-  __shared__ C twl[L];
-  const int slot = bt.slots[blockIdx.y];
-  work += (long long)slot * bt.stride;
-  psi += (long long)slot * bt.stride;
-  partial += (long long)slot * bt.pstride;
-  const Phys& ph = bt.ia[slot].ph;
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const int ntiles = segs.tiles0 + segs.tiles1;
   double acc[XPART] = {0.0, 0.0};
   // both input streams are read up front (their latency hides behind the
   // forward transform), the two output streams written in bursts after it
   XTile<T, L, E> x;
-  xload<T, L, TC>(x, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), ntiles, ci, t, work, psi);
+  xload<T, L, TC>(x, segs, tile, ntiles, ci, t, work, psi);
   stage_table<NT>(twl, tw, L);  // read after the first exchange's barrier
   xtile<T, L, TC, SAMPLE, LUT>(x, ci, t, work, psi, lds, twl, g, ph, acc);
-  if constexpr (SAMPLE) block_sum<NT, XPART>(acc, partial + (long long)blockIdx.x * XPART);
+  if constexpr (SAMPLE) block_sum<NT, XPART>(acc, partial + (long long)wb * XPART);
+}
+
+template <typename T, int L, int TC, bool SAMPLE, bool LUT>
+__global__ __launch_bounds__((tile_threads<L, TC>()))
+void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
+             const cpx<T>* __restrict__ tw, KGeom g, Batch bt, double* __restrict__ partial) {
+  using C = cpx<T>;
+  __shared__ C lds[L * (TC + PADT)];
+  __shared__ C twl[L];
+  const int slot = bt.slots[blockIdx.y];
+  work += (long long)slot * bt.stride;
+  psi += (long long)slot * bt.stride;
+  partial += (long long)slot * bt.pstride;
+  xstep_body<T, L, TC, SAMPLE, LUT>(work, psi, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), tw, g,
+                                    bt.ia[slot].ph, lds, twl, partial, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -360,35 +378,32 @@ __device__ __forceinline__ void with_powers(const ZArgs& za, F&& body) {
 // power of two <= 64, so a line never straddles two wavefronts).
 // FREE: no occupancy hint (the compiler may use more registers instead of
 // spilling); a register-allocation variant only, the arithmetic is identical.
-template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE, bool PW>
-__global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
-void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
-             cpx<T>* __restrict__ out_ny, long long nlines, ZLines zl, Batch bt,
-             const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
-             double* __restrict__ partial) {
+// LDS of the real-space pass: G lines of z_len<H> plus the two H-entry tables.
+template <typename T, int H, int NTZ> constexpr int z_lds_elems() {
+  return (NTZ / z_tpl<T, H>()) * z_len<H>() + 2 * H;
+}
+
+// The lines of workgroup `wb` of one instance (pointers already at the
+// instance's regions); `lds` holds the line buffer and, after it, the tables.
+template <typename T, int H, int NTZ, int MODE, bool SAMPLE, bool PW>
+__device__ __forceinline__ void zpass_body(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny,
+                                           void* __restrict__ out, cpx<T>* __restrict__ out_ny,
+                                           long long nlines, const ZLines& zl, const ZArgs& za,
+                                           const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
+                                           double* __restrict__ partial, cpx<T>* lds, int wb) {
   using C = cpx<T>;
-  const int slot = bt.slots[blockIdx.y];
-  {
-    const long long so = (long long)slot * bt.stride;  // complex elements
-    in = reinterpret_cast<const C*>(in) + so;
-    out = reinterpret_cast<C*>(out) + so;
-    if (in_ny) in_ny += so;
-    if (out_ny) out_ny += so;
-    partial += (long long)slot * bt.pzstride;
-  }
-  const ZArgs za = bt.ia[slot].za;
   constexpr int E = z_elems<T, H>();
   constexpr int TPL = z_tpl<T, H>();
   static_assert(TPL <= 64 && (TPL & (TPL - 1)) == 0, "a line must lie in one wavefront");
   constexpr int G = NTZ / TPL;
-  __shared__ C lds[G * z_len<H>()];
-  __shared__ C twl[H], twpl[H];  // the H-point FFT twiddles and the half-spectrum twiddles
+  C* twl = lds + G * z_len<H>();  // the H-point FFT twiddles
+  C* twpl = twl + H;              // and the half-spectrum twiddles
   stage_table<NTZ>(twl, tw, H);
   stage_table<NTZ>(twpl, twp, H);
   __syncthreads();
   const int tid = threadIdx.x;
   const int g = tid / TPL, t = tid - g * TPL;
-  const long long line = (long long)blockIdx.x * G + g;
+  const long long line = (long long)wb * G + g;
   const bool act = line < nlines;
   const int lg1 = __builtin_ctz(zl.n1);  // n1 is a power of two
   const long long la = line >> lg1, lb = line & (zl.n1 - 1);
@@ -438,7 +453,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
           if (act) rp[k] = mk<C>((T)p0, (T)p1);
         }
       });
-      if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)blockIdx.x * ZPART);
+      if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)wb * ZPART);
       return;
     } else {
       const T a = (T)za.a, b = (T)za.b;
@@ -499,7 +514,23 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
       }
     }
   }
-  if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)blockIdx.x * ZPART);
+  if constexpr (SAMPLE) block_sum<NTZ, ZPART>(acc, partial + (long long)wb * ZPART);
+}
+
+template <typename T, int H, int NTZ, bool FREE, int MODE, bool SAMPLE, bool PW>
+__global__ __launch_bounds__(NTZ, (FREE ? 1 : (sizeof(T) == 4 ? 4 : 2)))
+void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void* __restrict__ out,
+             cpx<T>* __restrict__ out_ny, long long nlines, ZLines zl, Batch bt,
+             const cpx<T>* __restrict__ tw, const cpx<T>* __restrict__ twp,
+             double* __restrict__ partial) {
+  using C = cpx<T>;
+  __shared__ C lds[z_lds_elems<T, H, NTZ>()];
+  const int slot = bt.slots[blockIdx.y];
+  const long long so = (long long)slot * bt.stride;  // complex elements
+  zpass_body<T, H, NTZ, MODE, SAMPLE, PW>(reinterpret_cast<const C*>(in) + so, in_ny ? in_ny + so : nullptr,
+                                          reinterpret_cast<C*>(out) + so, out_ny ? out_ny + so : nullptr,
+                                          nlines, zl, bt.ia[slot].za, tw, twp,
+                                          partial + (long long)slot * bt.pzstride, lds, blockIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -840,6 +871,228 @@ int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream
   if (mcut <= 0) return 0;
   hipLaunchKernelGGL(k_c2table, dim3((mcut + 255) / 256), dim3(256), 0, s, tab, mcut, dksq, ph);
   return launch_status();
+}
+
+// ---------------------------------------------------------------------------
+// ragged launches: several size classes in one launch per pass
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int rag_class(const RClass* __restrict__ cls, int ncls) {
+  int k = 0;
+  for (int i = 1; i < ncls; ++i)
+    if ((int)blockIdx.x >= cls[i].wg0) k = i;
+  return k;
+}
+
+#define XPFC_RLSWITCH(L, ...)                                    \
+  switch (L) {                                                   \
+    case 8: { constexpr int LL = 8; __VA_ARGS__; } break;        \
+    case 16: { constexpr int LL = 16; __VA_ARGS__; } break;      \
+    case 32: { constexpr int LL = 32; __VA_ARGS__; } break;      \
+    case 64: { constexpr int LL = 64; __VA_ARGS__; } break;      \
+    case 128: { constexpr int LL = 128; __VA_ARGS__; } break;    \
+    case 256: { constexpr int LL = 256; __VA_ARGS__; } break;    \
+    case 512: { constexpr int LL = 512; __VA_ARGS__; } break;    \
+    case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break;  \
+    default: break;                                              \
+  }
+
+// LUT: every class of the launch has a square / cubic box (C2 from the
+// table); the other form evaluates C2 per mode and needs more registers, so
+// the two never share a kernel.
+template <typename T, bool SAMPLE, bool LUT>
+__global__ __launch_bounds__(RAG_NT) void k_xstep_r(const RClass* __restrict__ cls, int ncls, int sib) {
+  using C = cpx<T>;
+  extern __shared__ __align__(16) unsigned char rag_smem[];
+  const RClass& c = cls[rag_class(cls, ncls)];
+  const int n = c.nact * c.wgs;
+  int local = (int)blockIdx.x - c.wg0;
+  if (local >= n) return;  // padding up to the next class
+  local = xcd_tile(local, n, c.lgc);
+  const int ii = local / c.wgs, tile = local - ii * c.wgs;
+  const int slot = c.bt.slots[ii];
+  if (c.budget && sib >= c.budget[slot]) return;
+  C* work = (C*)c.a + (long long)slot * c.bt.stride;
+  C* psi = (C*)c.b + (long long)slot * c.bt.stride;
+  double* part = c.partial + (long long)slot * c.bt.pstride;
+  const Phys& ph = c.bt.ia[slot].ph;
+  C* lds = reinterpret_cast<C*>(rag_smem);
+  XPFC_RLSWITCH(c.L, {
+    constexpr int TC = rag_tile(LL);
+    if constexpr (tile_ok<T, LL, TC>()) {
+      xstep_body<T, LL, TC, SAMPLE, LUT>(work, psi, c.segs, tile, (const C*)c.tw, c.g, ph, lds,
+                                         lds + LL * (TC + PADT), part, tile);
+    }
+  })
+}
+
+template <typename T, int DIR>
+__global__ __launch_bounds__(RAG_NT) void k_strided_r(const RClass* __restrict__ cls, int ncls, int sib) {
+  using C = cpx<T>;
+  extern __shared__ __align__(16) unsigned char rag_smem[];
+  const RClass& c = cls[rag_class(cls, ncls)];
+  const int n = c.nact * c.wgs;
+  int local = (int)blockIdx.x - c.wg0;
+  if (local >= n) return;  // padding up to the next class
+  local = xcd_tile(local, n, c.lgc);
+  const int ii = local / c.wgs, tile = local - ii * c.wgs;
+  const int slot = c.bt.slots[ii];
+  if (c.budget && sib >= c.budget[slot]) return;
+  C* buf = (C*)c.a + (long long)slot * c.bt.stride;
+  C* lds = reinterpret_cast<C*>(rag_smem);
+  XPFC_RLSWITCH(c.L, {
+    constexpr int TC = rag_tile(LL);
+    if constexpr (tile_ok<T, LL, TC>())
+      strided_body<T, LL, TC, DIR, false>(buf, c.segs, tile, (const C*)c.tw, lds, lds + LL * (TC + PADT));
+  })
+}
+
+// No occupancy hint: the bodies of every line length share one register
+// allocation, which a 4-waves/SIMD bound would make spill.
+template <typename T, bool SAMPLE, bool PW>
+__global__ __launch_bounds__(RAG_NTZ)
+void k_zpass_r(const RClass* __restrict__ cls, int ncls, int sib) {
+  using C = cpx<T>;
+  extern __shared__ __align__(16) unsigned char rag_smem[];
+  const RClass& c = cls[rag_class(cls, ncls)];
+  const int local = (int)blockIdx.x - c.wg0;
+  const int ii = local / c.wgs, wb = local - ii * c.wgs;
+  if (ii >= c.nact) return;  // padding up to the next class
+  const int slot = c.bt.slots[ii];
+  if (c.budget && sib >= c.budget[slot]) return;
+  C* base = (C*)c.a + (long long)slot * c.bt.stride;
+  C* ny = base + c.nyoff;
+  double* part = c.partial + (long long)slot * c.bt.pzstride;
+  C* lds = reinterpret_cast<C*>(rag_smem);
+  // H = n2 / 2 <= 512 on every supported grid
+#define XPFC_RZ(HH)                                                                               \
+  case HH:                                                                                        \
+    if constexpr (z_ok<T, HH, RAG_NTZ>())                                                         \
+      zpass_body<T, HH, RAG_NTZ, Z_STEP, SAMPLE, PW>(base, ny, base, ny, c.nlines, c.zl,           \
+                                                      c.bt.ia[slot].za, (const C*)c.tw,           \
+                                                      (const C*)c.twp, part, lds, wb);            \
+    break;
+  switch (c.L) {
+    XPFC_RZ(4) XPFC_RZ(8) XPFC_RZ(16) XPFC_RZ(32) XPFC_RZ(64) XPFC_RZ(128) XPFC_RZ(256) XPFC_RZ(512)
+    default: break;
+  }
+#undef XPFC_RZ
+}
+
+namespace {
+// dynamic LDS beyond the default limit needs the per-kernel attribute
+template <class K>
+bool allow_lds(K kernel, size_t bytes) {
+  if (bytes <= 65536) return true;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                             (int)bytes) == hipSuccess;
+}
+template <typename T>
+int rag_prepare_T(size_t lx, size_t ly, size_t lz) {
+  const bool ok = allow_lds(k_xstep_r<T, true, true>, lx) && allow_lds(k_xstep_r<T, false, true>, lx) &&
+                  allow_lds(k_xstep_r<T, true, false>, lx) && allow_lds(k_xstep_r<T, false, false>, lx) &&
+                  allow_lds(k_strided_r<T, -1>, ly) && allow_lds(k_strided_r<T, 1>, ly) &&
+                  allow_lds(k_zpass_r<T, true, true>, lz) && allow_lds(k_zpass_r<T, true, false>, lz) &&
+                  allow_lds(k_zpass_r<T, false, true>, lz) && allow_lds(k_zpass_r<T, false, false>, lz);
+  return ok ? 0 : launch_status(-2);
+}
+}  // namespace
+
+int rag_prepare(int prec, size_t lx, size_t ly, size_t lz) {
+  return prec == 1 ? rag_prepare_T<float>(lx, ly, lz) : rag_prepare_T<double>(lx, ly, lz);
+}
+
+template <typename T, bool LUT>
+static int xr_T(const RClass* cls, int ncls, int nwg, size_t lds, bool sample, int sib, hipStream_t s) {
+  if (nwg <= 0) return 0;
+  if (sample)
+    hipLaunchKernelGGL((k_xstep_r<T, true, LUT>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
+  else
+    hipLaunchKernelGGL((k_xstep_r<T, false, LUT>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
+  return launch_status();
+}
+
+int launch_xstep_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool lut, bool sample,
+                        int sib, hipStream_t s) {
+  if (prec == 1)
+    return lut ? xr_T<float, true>(cls, ncls, nwg, lds, sample, sib, s)
+               : xr_T<float, false>(cls, ncls, nwg, lds, sample, sib, s);
+  return lut ? xr_T<double, true>(cls, ncls, nwg, lds, sample, sib, s)
+             : xr_T<double, false>(cls, ncls, nwg, lds, sample, sib, s);
+}
+
+template <typename T>
+static int yr_T(const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib, hipStream_t s) {
+  if (nwg <= 0) return 0;
+  if (dir < 0)
+    hipLaunchKernelGGL((k_strided_r<T, -1>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
+  else
+    hipLaunchKernelGGL((k_strided_r<T, 1>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
+  return launch_status();
+}
+
+int launch_strided_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib,
+                          hipStream_t s) {
+  return prec == 1 ? yr_T<float>(cls, ncls, nwg, lds, dir, sib, s) : yr_T<double>(cls, ncls, nwg, lds, dir, sib, s);
+}
+
+template <typename T, bool SAMPLE, bool PW>
+static int zr_go(const RClass* cls, int ncls, int nwg, size_t lds, int sib, hipStream_t s) {
+  hipLaunchKernelGGL((k_zpass_r<T, SAMPLE, PW>), dim3(nwg), dim3(RAG_NTZ), lds, s, cls, ncls, sib);
+  return launch_status();
+}
+
+template <typename T>
+static int zr_T(const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw, int sib, hipStream_t s) {
+  if (nwg <= 0) return 0;
+  if (sample) return pw ? zr_go<T, true, true>(cls, ncls, nwg, lds, sib, s) : zr_go<T, true, false>(cls, ncls, nwg, lds, sib, s);
+  return pw ? zr_go<T, false, true>(cls, ncls, nwg, lds, sib, s) : zr_go<T, false, false>(cls, ncls, nwg, lds, sib, s);
+}
+
+int launch_zpass_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw,
+                        int sib, hipStream_t s) {
+  return prec == 1 ? zr_T<float>(cls, ncls, nwg, lds, sample, pw, sib, s)
+                   : zr_T<double>(cls, ncls, nwg, lds, sample, pw, sib, s);
+}
+
+template <typename T>
+static int rsw_T(int L, const Segs& segs) {
+  XPFC_RLSWITCH(L, {
+    constexpr int TC = rag_tile(LL);
+    if constexpr (tile_ok<T, LL, TC>()) {
+      const Segs sg = fill_tiles(segs, TC);
+      return sg.tiles0 + sg.tiles1;
+    }
+    return -3;
+  })
+  return -1;
+}
+int rag_strided_wgs(int prec, int L, const Segs& segs) {
+  return prec == 1 ? rsw_T<float>(L, segs) : rsw_T<double>(L, segs);
+}
+size_t rag_strided_lds(int prec, int L) {
+  if (L < 8) return 0;
+  return ((size_t)L * (rag_tile(L) + PADT) + L) * (prec == 1 ? 8 : 16);
+}
+
+template <typename T>
+static long long rzw_T(int H, long long nlines, size_t* lds) {
+  XPFC_LSWITCH(H, {
+    if constexpr (z_ok<T, LL, RAG_NTZ>()) {
+      constexpr int G = RAG_NTZ / z_tpl<T, LL>();
+      if (lds) *lds = (size_t)z_lds_elems<T, LL, RAG_NTZ>() * sizeof(cpx<T>);
+      return (nlines + G - 1) / G;
+    }
+    return -3;
+  })
+  return -1;
+}
+int rag_zpass_wgs(int prec, int H, long long nlines) {
+  return (int)(prec == 1 ? rzw_T<float>(H, nlines, nullptr) : rzw_T<double>(H, nlines, nullptr));
+}
+size_t rag_zpass_lds(int prec, int H) {
+  size_t b = 0;
+  if ((prec == 1 ? rzw_T<float>(H, 1, &b) : rzw_T<double>(H, 1, &b)) < 0) return 0;
+  return b;
 }
 
 }  // namespace xpfc
