@@ -244,11 +244,12 @@ def batched_run(steps=2000, ninst=64, seed=0):
 
 
 def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 << 25,
-                lookahead=1):
+                lookahead=1, lanes="class"):
     """Poisson job stream through the continuous-batching service."""
     from xpfc_cuda_amd import serving as S
     jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d)
-    svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells), lookahead=lookahead)
+    svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells), lookahead=lookahead,
+                    lanes=S.lane_groups(lanes))
     try:
         svc.warm([(2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128), (3, 256)])
         rep = svc.run(jobs, realtime=True,
@@ -257,7 +258,7 @@ def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 
         offered = sum(j.cells * j.steps for j in jobs) / seconds
         rep.update({"arrival_rate": rate, "window_s": seconds, "frac_3d": frac_3d,
                     "offered_cell_updates_per_s": offered,
-                    "budget_gb": budget / 1e9, "lookahead_blocks": lookahead,
+                    "budget_gb": budget / 1e9, "lookahead_blocks": lookahead, "lanes": lanes,
                     "rounding": "side log-uniform, rounded to the nearest power of two in log2",
                     "steps": "U{500..5000}", "sigma": "U[0.05,0.2]", "n0": "U[0.02,0.08]"})
     finally:
@@ -284,6 +285,10 @@ def main():
                     help="slot policy: cells per size-class launch")
     ap.add_argument("--serve-lookahead", type=int, default=1,
                     help="blocks handed at once to a size class no queued job waits for")
+    ap.add_argument("--serve-lanes", default="class",
+                    help="'class' (a lane, with its own stream and block cadence, per size class), "
+                         "'one' (one lane for all), or groups of classes such as "
+                         "'2d_128,2d_256;2d_512,2d_1024;3d_64,3d_128;3d_256'")
     ap.add_argument("--only", choices=["batch", "serve", "configs"],
                     help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -320,7 +325,8 @@ def main():
     if args.only == "serve":
         print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d,
                                                   class_cells=args.serve_class_cells,
-                                                  lookahead=args.serve_lookahead)}), flush=True)
+                                                  lookahead=args.serve_lookahead,
+                                                  lanes=args.serve_lanes)}), flush=True)
         return
     n = args.n
     pin = [int(v) for v in args.variants.split(",")] if args.variants else None
@@ -391,7 +397,7 @@ def main():
     if args.serve > 0:
         out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d,
                                      class_cells=args.serve_class_cells,
-                                     lookahead=args.serve_lookahead)
+                                     lookahead=args.serve_lookahead, lanes=args.serve_lanes)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

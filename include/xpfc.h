@@ -311,6 +311,100 @@ int xpfc_get_inst_params(const xpfc_ctx* ctx, int s, xpfc_params* out);
  * xpfc_time_passes refuse it (XPFC_ERR_ARG); xpfc_autotune has nothing to
  * choose (variants 0). */
 int xpfc_create_mixed(const xpfc_params* p, int ninst, xpfc_ctx** out);  /* p[ninst] */
+/* One block of len steps in which each active instance s takes steps[s]
+ * (0..len) of them, all in one captured graph (any context: one grid, a
+ * batch, or several grids): jobs of different remaining lengths advance
+ * together without splitting the block.  No instance may have a sampled step
+ * inside the block (XPFC_ERR_ARG; use xpfc_step for sampled runs).
+ * steps[ninst].  No counterpart in the reference. */
+int xpfc_step_budget(xpfc_ctx* ctx, int len, const int* steps);
+
+/* ---- the job service ------------------------------------------------------- */
+/* No counterpart in the reference.  Simulation jobs (a grid, physics, step
+ * count and noise initial condition) are queued, admitted in arrival order
+ * into free slots of their grid class while their device footprint
+ * (xpfc_instance_bytes) fits the budget beside the resident jobs (a job that
+ * does not fit blocks the later ones), and run in blocks of steps: the
+ * classes of one lane advance together, every block of a lane advancing all
+ * its running jobs with one launch per pass (a context of several grids), each
+ * job by min(block, its remaining steps).  Lanes are independent (each its own
+ * context and stream, with its own block cadence), so short jobs of a small
+ * grid are not held to the blocks of a large one.  A job whose field diverges
+ * stops at the end of that block.  Fields stay on the device
+ * from admission to completion; a job's result (and kept field) is what the
+ * same job gives alone in a one-instance context. */
+#define XPFC_JOB_DONE 0
+#define XPFC_JOB_NONFINITE 1      /* the field diverged; stopped early          */
+#define XPFC_JOB_REJECTED 2       /* unknown grid class, steps < 1 or footprint > budget */
+
+typedef struct xpfc_service xpfc_service;
+
+typedef struct xpfc_service_class {
+  int dim;                   /* 2 or 3                                          */
+  int n[3];                  /* grid extents (n[2] ignored in 2-D)              */
+  double L[3];               /* box lengths                                     */
+  int slots;                 /* jobs of this grid that can run at once          */
+  int lane;                  /* classes with equal lane numbers share blocks    */
+} xpfc_service_class;
+
+typedef struct xpfc_service_config {
+  int precision;             /* XPFC_FP64 or XPFC_FP32_MIXED                    */
+  int device;
+  double budget_bytes;       /* device bytes the resident jobs may hold         */
+  int block_steps;           /* steps per block while jobs wait (50)            */
+  int lookahead;             /* blocks per launch while no job waits (1)        */
+  int keep_fields;           /* 1: keep each finished job's field on the host   */
+  int nclasses;
+  const xpfc_service_class* classes;
+} xpfc_service_config;
+
+typedef struct xpfc_job {
+  long long id;
+  xpfc_params params;        /* grid (must match a class), physics; t_sample ignored */
+  long long steps;
+  double amplitude;          /* initial field psi0 + amplitude * randn(seed, cell) */
+  unsigned long long seed;
+  double arrival;            /* seconds on the service clock; < 0: now          */
+} xpfc_job;
+
+typedef struct xpfc_job_result {
+  long long id;
+  int status;                /* XPFC_JOB_*                                      */
+  long long steps_done;
+  double F_ref, f_mean, mean;  /* final free energy (xpfc_fe_calc) and mean     */
+  double t_arrival, t_admitted, t_finished;  /* seconds on the service clock   */
+} xpfc_job_result;
+
+typedef struct xpfc_service_stats {
+  long long submitted, rejected, done, nonfinite, queued, running, blocks, queue_max;
+  double resident_bytes, peak_resident_bytes;
+  double cell_updates;       /* cell updates run so far                         */
+  double elapsed;            /* seconds since creation                          */
+} xpfc_service_stats;
+
+/* sizeof of the structs above (which = 0..4: class, config, job, result,
+ * stats), for bindings that mirror them. */
+size_t xpfc_service_struct_size(int which);
+int xpfc_service_create(const xpfc_service_config* cfg, xpfc_service** out);
+void xpfc_service_destroy(xpfc_service* svc);
+/* Queue a job; XPFC_ERR_ARG (and a rejected result) if it cannot ever run. */
+int xpfc_service_submit(xpfc_service* svc, const xpfc_job* job);
+/* One non-blocking round (finish the lanes' blocks that are done, admit,
+ * launch the next block of every lane between blocks): 1 if anything
+ * happened, 0 if not, minus a status code. */
+int xpfc_service_poll(xpfc_service* svc);
+/* 1 when nothing is queued, running or in flight. */
+int xpfc_service_idle(const xpfc_service* svc);
+/* Serve `n` jobs: submit each at its arrival time (realtime = 1) or all at
+ * once (0), poll until every job has finished.  Blocks. */
+int xpfc_service_run(xpfc_service* svc, const xpfc_job* jobs, int n, int realtime);
+/* Results of the jobs finished since the last call (returns the count copied,
+ * at most max; the rest stay), minus a status code on failure. */
+int xpfc_service_results(xpfc_service* svc, xpfc_job_result* out, int max);
+/* The kept field of finished job `id` (keep_fields), capacity values;
+ * returns the number of cells, minus a status code. */
+long long xpfc_service_field(xpfc_service* svc, long long id, double* host, long long capacity);
+int xpfc_service_get_stats(const xpfc_service* svc, xpfc_service_stats* out);
 
 /* ---- errors ----------------------------------------------------------------- */
 const char* xpfc_strerror(int status);

@@ -133,3 +133,74 @@ def test_mixed_rejects_operator_calls_and_mixed_precision():
         mc.initialize_noise(0, 0.01, 1)
         with pytest.raises(XPFCError):
             mc.step(1)                    # instance 1 has no field yet
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+def test_budgeted_blocks_equal_alone(prec):
+    """xpfc_step_budget: one block in which each instance takes its own number
+    of steps (0 .. the block length), twice in a row with other budgets; every
+    field equals its run alone over the same total, and sampled steps inside a
+    block are refused."""
+    import ctypes as ct
+    from xpfc_cuda_amd import XPFCError, lib
+    _flush("budgeted blocks prec=%d" % prec)
+    ps = []
+    for p in _mix(prec):
+        p.t_sample = 0
+        ps.append(p)
+    psis = [_field(tuple(p.n), 70 + s) for s, p in enumerate(ps)]
+    budgets = [[50, 0, 17, 32, 1, 50, 9], [20, 32, 32, 3, 32, 0, 31]]
+    lens = [50, 32]
+    with MixedContext(ps) as mc:
+        for s, x in enumerate(psis):
+            mc.set_field(s, x)
+        for b, n in zip(budgets, lens):
+            arr = (ct.c_int * len(ps))(*b)
+            assert lib().xpfc_step_budget(mc.handle, n, arr) == 0
+        mc.sync()
+        totals = [a + b for a, b in zip(*budgets)]
+        assert [mc.current_step(s) for s in range(len(ps))] == totals
+        got = [mc.get_field(s) for s in range(len(ps))]
+        _, q = make((64, 64), prec=prec, t_sample=10)
+        mc.set_params(0, q)
+        arr = (ct.c_int * len(ps))(*([20] + [0] * (len(ps) - 1)))
+        assert lib().xpfc_step_budget(mc.handle, 20, arr) != 0   # steps 70 and 79 sample
+    for s, p in enumerate(ps):
+        a, _, _ = _alone(p, psis[s], totals[s])
+        assert np.array_equal(got[s], a), "instance %d differs" % s
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_budgeted_blocks_one_grid(prec, graphs):
+    """xpfc_step_budget on contexts of one grid (a batch, 3-D and 2-D): each
+    active instance takes its own number of the block's steps; two blocks
+    with other budgets and another active list; every field equals its run
+    alone."""
+    import ctypes as ct
+    from xpfc_cuda_amd import BatchContext, lib
+    _flush("budgeted one-grid blocks prec=%d graphs=%d" % (prec, graphs))
+    for shape in [(16, 16, 32), (64, 64)]:
+        ps = []
+        for s in range(5):
+            _, p = make(shape, prec=prec, sigma=0.06 + 0.02 * s, psi0=0.03 + 0.01 * s, t_sample=0)
+            ps.append(p)
+        psis = [_field(shape, 90 + s) for s in range(5)]
+        budgets = [[16, 0, 9, 16, 1], [5, 32, 0, 31, 32]]
+        lens = [16, 32]
+        with BatchContext(ps) as bc:
+            bc.set_graphs(graphs)
+            for s, x in enumerate(psis):
+                bc.set_field(s, x)
+            arr = (ct.c_int * 5)(*budgets[0])
+            assert lib().xpfc_step_budget(bc.handle, lens[0], arr) == 0
+            bc.set_active([4, 1, 3, 0])   # instance 2 sits out the second block
+            arr = (ct.c_int * 5)(*budgets[1])
+            assert lib().xpfc_step_budget(bc.handle, lens[1], arr) == 0
+            bc.sync()
+            totals = [budgets[0][s] + (budgets[1][s] if s != 2 else 0) for s in range(5)]
+            assert [bc.current_step(s) for s in range(5)] == totals
+            got = [bc.get_field(s) for s in range(5)]
+        for s, p in enumerate(ps):
+            a, _, _ = _alone(p, psis[s], totals[s])
+            assert np.array_equal(got[s], a), "%s instance %d differs" % (shape, s)

@@ -61,3 +61,38 @@ def test_alone_equals_mix(lookahead):
         assert job.status == "done" and np.all(np.isfinite(alone))
         assert np.array_equal(job.field, alone), "job %d differs from its solo run" % job.jid
         assert job.F_ref == F and job.f_mean == fm and job.mean == m
+
+
+def test_service_run_in_the_library():
+    """xpfc_service_run: the library's own serving loop (arrival times, polling)
+    over a few jobs of two classes, one of which diverges; the results come
+    back through xpfc_service_results and match each job alone."""
+    import ctypes as ct
+    from xpfc_cuda_amd import lib
+    from xpfc_cuda_amd.api import _Job, _JobResult
+    print("service run in the library", flush=True)
+    jobs = _jobs()[:6]
+    svc = S.Service(slots=lambda d, s: 2, keep_fields=True)
+    svc.warm(sorted({(j.dim, j.side) for j in jobs}))
+    arr = (_Job * len(jobs))(*[svc._request(j, 0.01 * j.jid) for j in jobs])
+    try:
+        assert lib().xpfc_service_run(svc.handle, arr, len(jobs), 1) == 0
+        out = (_JobResult * 16)()
+        n = lib().xpfc_service_results(svc.handle, out, 16)
+        assert n == len(jobs)
+        res = {r.id: r for r in out[:n]}
+        st = svc.stats()
+        assert st["done"] + st["nonfinite"] == len(jobs) and st["nonfinite"] == 1
+        for j in jobs:
+            r = res[j.jid]
+            assert r.t_arrival <= r.t_admitted <= r.t_finished
+            with Context(S.job_params(j)) as c:
+                c.initialize_noise(j.amplitude, j.seed)
+                c.step(j.steps)
+                if j.jid == 4:
+                    assert r.status == 1 and r.steps_done < j.steps
+                    continue
+                F, _ = c.fe_calc()
+            assert r.status == 0 and r.steps_done == j.steps and r.F_ref == F
+    finally:
+        svc.close()

@@ -62,6 +62,35 @@ class _Sample(ct.Structure):
                 ("rmse", ct.c_double)]
 
 
+class _ServiceClass(ct.Structure):
+    _fields_ = [("dim", ct.c_int), ("n", ct.c_int * 3), ("L", ct.c_double * 3), ("slots", ct.c_int),
+                ("lane", ct.c_int)]
+
+
+class _ServiceConfig(ct.Structure):
+    _fields_ = [("precision", ct.c_int), ("device", ct.c_int), ("budget_bytes", ct.c_double),
+                ("block_steps", ct.c_int), ("lookahead", ct.c_int), ("keep_fields", ct.c_int),
+                ("nclasses", ct.c_int), ("classes", ct.POINTER(_ServiceClass))]
+
+
+class _Job(ct.Structure):
+    _fields_ = [("id", ct.c_longlong), ("params", _Params), ("steps", ct.c_longlong),
+                ("amplitude", ct.c_double), ("seed", ct.c_ulonglong), ("arrival", ct.c_double)]
+
+
+class _JobResult(ct.Structure):
+    _fields_ = [("id", ct.c_longlong), ("status", ct.c_int), ("steps_done", ct.c_longlong),
+                ("F_ref", ct.c_double), ("f_mean", ct.c_double), ("mean", ct.c_double),
+                ("t_arrival", ct.c_double), ("t_admitted", ct.c_double), ("t_finished", ct.c_double)]
+
+
+class _ServiceStats(ct.Structure):
+    _fields_ = [(k, ct.c_longlong) for k in ("submitted", "rejected", "done", "nonfinite", "queued",
+                                             "running", "blocks", "queue_max")] + \
+               [(k, ct.c_double) for k in ("resident_bytes", "peak_resident_bytes", "cell_updates",
+                                           "elapsed")]
+
+
 _lib = None
 
 
@@ -128,6 +157,17 @@ def load_library(path: Optional[str] = None):
         "xpfc_load_checkpoint": (i, [vp, ct.c_char_p]),
         "xpfc_create_batch": (i, [P(_Params), i, P(vp)]),
         "xpfc_create_mixed": (i, [P(_Params), i, P(vp)]),
+        "xpfc_step_budget": (i, [vp, i, P(i)]),
+        "xpfc_service_struct_size": (sz, [i]),
+        "xpfc_service_create": (i, [P(_ServiceConfig), P(vp)]),
+        "xpfc_service_destroy": (None, [vp]),
+        "xpfc_service_submit": (i, [vp, P(_Job)]),
+        "xpfc_service_poll": (i, [vp]),
+        "xpfc_service_idle": (i, [vp]),
+        "xpfc_service_run": (i, [vp, P(_Job), i, i]),
+        "xpfc_service_results": (i, [vp, P(_JobResult), i]),
+        "xpfc_service_field": (ll, [vp, ll, P(d), ll]),
+        "xpfc_service_get_stats": (i, [vp, P(_ServiceStats)]),
         "xpfc_get_inst_params": (i, [vp, i, P(_Params)]),
         "xpfc_num_instances": (i, [vp]),
         "xpfc_instance_bytes": (sz, [P(_Params)]),
@@ -159,6 +199,10 @@ def load_library(path: Optional[str] = None):
     if L.xpfc_params_size() != ct.sizeof(_Params):
         raise ImportError("xpfc_params layout mismatch: library %d bytes, binding %d"
                           % (L.xpfc_params_size(), ct.sizeof(_Params)))
+    for which, st in enumerate((_ServiceClass, _ServiceConfig, _Job, _JobResult, _ServiceStats)):
+        if L.xpfc_service_struct_size(which) != ct.sizeof(st):
+            raise ImportError("%s layout mismatch: library %d bytes, binding %d"
+                              % (st.__name__, L.xpfc_service_struct_size(which), ct.sizeof(st)))
     _lib = L
     return L
 

@@ -27,7 +27,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contra
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp"]
+SOURCES = ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp", "xpfc_sched.cpp",
+           "xpfc_service.cpp"]
 
 
 def _hipcc() -> str:

@@ -276,7 +276,7 @@ int upload_inst(xpfc_ctx* c, int s) {
 
 Batch bt_of(const xpfc_ctx* c, const int* slots) {
   return Batch{slots, c->d_ia, c->stride, (long long)c->npx_cap * XPART,
-               (long long)c->npz_cap * ZPART};
+               (long long)c->npz_cap * ZPART, nullptr, 0};
 }
 Batch bt_active(const xpfc_ctx* c) { return bt_of(c, c->d_active); }
 Batch bt_one(const xpfc_ctx* c, int s) { return bt_of(c, c->d_ids + s); }
@@ -427,6 +427,7 @@ int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out
     return bail(fail(XPFC_ERR_NOMEM, "table allocation failed"));
   try {
     c->hsums.assign((size_t)SUM_ROWS * K * NSUMS, 0.0);  // read-back rows of drain
+    c->hbud.assign((size_t)K, 0);
   } catch (const std::bad_alloc&) {
     return bail(fail(XPFC_ERR_NOMEM, "host allocation failed"));
   }
@@ -914,9 +915,14 @@ bool inst_samples(const XInst& in, long long n) {
 
 // The launches of one step (four in 3-D, two in 2-D) over every active
 // instance.  With `ev`, an event is recorded after each pass for timing.
-int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr) {
+// With `budget`, this is step `sib` of a budgeted block: instances whose
+// budget is used up skip it.
+int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr, const int* budget = nullptr,
+                    int sib = 0) {
   const long long nl = (long long)c->n0 * c->n1;
-  const Batch bt = bt_active(c);
+  Batch bt = bt_active(c);
+  bt.budget = budget;
+  bt.sib = sib;
   const int ny = (int)c->active.size();
   const int nbx = launch_xstep(c->prec, c->work, c->psi, x_segs(c, c->tc_x), c->n0, c->tc_x, c->tw0, c->g, bt, ny,
                                c->px, sample, c->stream);
@@ -939,11 +945,13 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr)
 // or a power of two below it, cached per context by (length, number of active
 // instances): the kernels read the active list from device memory, so a new
 // list of the same length reuses the graphs.  The captured work is the same
-// launch sequence as an eager step, so the fields are identical.
-int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
+// launch sequence as an eager step, so the fields are identical.  Budgeted
+// blocks (per-instance budgets read from d_budget) are cached under -len.
+int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out, bool budgeted = false) {
   const int ny = (int)c->active.size();
+  const int key = budgeted ? -len : len;
   for (auto& g : c->graphs)
-    if (g.first.first == len && g.first.second == ny) {
+    if (g.first.first == key && g.first.second == ny) {
       *out = g.second;
       return XPFC_OK;
     }
@@ -951,7 +959,8 @@ int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
   hipGraph_t graph = nullptr;
   HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
   int rc = XPFC_OK;
-  for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = launch_one_step(c, false, 0);
+  for (int i = 0; i < len && rc == XPFC_OK; ++i)
+    rc = launch_one_step(c, false, 0, nullptr, budgeted ? c->d_budget : nullptr, i);
   const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
   if (rc != XPFC_OK || ec != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -961,13 +970,32 @@ int plain_graph(xpfc_ctx* c, int len, hipGraphExec_t* out) {
   const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
   (void)hipGraphDestroy(graph);
   if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
-  c->graphs.push_back({{len, ny}, ex});
+  c->graphs.push_back({{key, ny}, ex});
   *out = ex;
   return XPFC_OK;
 }
 
 void advance(xpfc_ctx* c, long long n) {
   for (int s : c->active) c->inst[s].step += n;
+}
+
+// One block of `len` steps in which active instance s takes hbud[s] of them
+// (one-grid contexts; the kernels of steps past an instance's budget return
+// at once): the budgets go to the device, then the block runs as one
+// captured graph.
+int uniform_budget_block(xpfc_ctx* c, int len) {
+  HIPCHK(hipMemcpyAsync(c->d_budget, c->hbud.data(), c->hbud.size() * sizeof(int), hipMemcpyHostToDevice,
+                        c->stream));
+  if (!c->use_graphs) {
+    for (int i = 0; i < len; ++i)
+      if (int rc = launch_one_step(c, false, 0, nullptr, c->d_budget, i)) return rc;
+  } else {
+    hipGraphExec_t ex = nullptr;
+    if (int rc = plain_graph(c, len, &ex, true)) return rc;
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+  }
+  for (int s : c->active) c->inst[s].step += c->hbud[(size_t)s];
+  return XPFC_OK;
 }
 
 int run_plain_steps(xpfc_ctx* c, long long run) {
@@ -2041,6 +2069,51 @@ int mixed_plain(xpfc_ctx* c, const xpfc_ctx::RagTables& t, long long run, bool p
   return XPFC_OK;
 }
 
+// One block of `len` steps in which slot i of class k takes hbud[i] of them
+// (the kernels of steps past an instance's budget return at once): the
+// budgets go to the device, then the block runs as one captured graph.
+int mixed_budget_block(xpfc_ctx* c, int len) {
+  const xpfc_ctx::RagTables* t = nullptr;
+  if (int rc = mixed_tables(c, true, &t)) return rc;
+  const bool pw = mixed_pw(c);
+  for (xpfc_ctx* q : c->cls)
+    HIPCHK(hipMemcpyAsync(q->d_budget, q->hbud.data(), q->hbud.size() * sizeof(int), hipMemcpyHostToDevice,
+                          c->stream));
+  if (!c->use_graphs) {
+    for (int i = 0; i < len; ++i)
+      if (int rc = mixed_one_step(c, *t, false, i, nullptr, pw)) return rc;
+  } else {
+    std::vector<int> key = active_counts(c);
+    key.push_back(len);
+    key.push_back(pw ? 1 : 0);
+    key.push_back(-1);  // budgeted block
+    hipGraphExec_t ex = nullptr;
+    auto it = c->rgraphs.find(key);
+    if (it != c->rgraphs.end()) {
+      ex = it->second;
+    } else {
+      if (c->rgraphs.size() >= 64) mixed_drop_graphs(c);
+      hipGraph_t graph = nullptr;
+      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      int rc = XPFC_OK;
+      for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = mixed_one_step(c, *t, false, i, nullptr, pw);
+      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      if (rc != XPFC_OK || ec != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc != XPFC_OK ? rc : fail(XPFC_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(ec));
+      }
+      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
+      c->rgraphs[key] = ex;
+    }
+    HIPCHK(hipGraphLaunch(ex, c->stream));
+  }
+  for (xpfc_ctx* q : c->cls)
+    for (int s : q->active) q->inst[s].step += q->hbud[s];
+  return XPFC_OK;
+}
+
 int mixed_step(xpfc_ctx* c, int nsteps) {
   if (nsteps == 0) return XPFC_OK;
   bool any = false;
@@ -2176,7 +2249,9 @@ extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
     for (int s : members[k]) gp.push_back(ps[s]);
     xpfc_ctx* q = nullptr;
     if (int rc = create_impl(gp.data(), (int)gp.size(), c->stream, &q)) return bail(rc);
+    q->hbud.assign((size_t)q->K, 0);
     c->cls.push_back(q);
+    c->inst_of.push_back(members[k]);
     for (size_t i = 0; i < members[k].size(); ++i) c->where[members[k][i]] = {(int)k, (int)i};
   }
   const char* eg = std::getenv("XPFC_GRAPHS");
@@ -2184,3 +2259,47 @@ extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
   *out = c;
   return XPFC_OK;
 }
+
+extern "C" int xpfc_step_budget(xpfc_ctx* c, int len, const int* steps) {
+  if (!c || !steps || len < 1) return fail(XPFC_ERR_ARG, "bad arguments");
+  if (c->cls.empty()) {  // one grid
+    bool any = false;
+    std::fill(c->hbud.begin(), c->hbud.end(), 0);
+    for (int s : c->active) {
+      const int b = steps[s];
+      const XInst& in = c->inst[(size_t)s];
+      if (b < 0 || b > len) return fail(XPFC_ERR_ARG, "instance %d: %d steps in a block of %d", s, b, len);
+      if (!in.have_field) return fail(XPFC_ERR_STATE, "instance %d: no field set", s);
+      for (long long n = in.step; n < in.step + b; ++n)
+        if (inst_samples(in, n))
+          return fail(XPFC_ERR_ARG, "instance %d samples at step %lld inside the block (use xpfc_step)", s, n);
+      c->hbud[(size_t)s] = b;
+      any = any || b > 0;
+    }
+    if (!any) return XPFC_OK;
+    if (!c->tuned && autotune_enabled()) {
+      if (int rc = autotune(c)) return rc;
+    }
+    return uniform_budget_block(c, len);
+  }
+  bool any = false;
+  for (size_t k = 0; k < c->cls.size(); ++k) {
+    xpfc_ctx* q = c->cls[k];
+    std::fill(q->hbud.begin(), q->hbud.end(), 0);
+    for (int slot : q->active) {
+      const int s = c->inst_of[k][(size_t)slot];
+      const int b = steps[s];
+      const XInst& in = q->inst[slot];
+      if (b < 0 || b > len) return fail(XPFC_ERR_ARG, "instance %d: %d steps in a block of %d", s, b, len);
+      if (!in.have_field) return fail(XPFC_ERR_STATE, "instance %d: no field set", s);
+      for (long long n = in.step; n < in.step + b; ++n)
+        if (inst_samples(in, n))
+          return fail(XPFC_ERR_ARG, "instance %d samples at step %lld inside the block (use xpfc_step)", s, n);
+      q->hbud[(size_t)slot] = b;
+      any = any || b > 0;
+    }
+  }
+  if (!any) return XPFC_OK;
+  return mixed_budget_block(c, len);
+}
+

@@ -106,12 +106,14 @@ struct xpfc_ctx {
   std::vector<double> hsums;         // host rows for drain: [SUM_ROWS][K][NSUMS]
   bool nonfinite_unreported = false;  // an instance diverged since the last report
   int* d_budget = nullptr;           // [K] steps of the current block per slot (mixed contexts)
+  std::vector<int> hbud;             // host copy of d_budget
 
   // ---- mixed-grid contexts (xpfc_create_mixed): instances of several grids.
   // Each grid is a class context holding that grid's instances (sharing this
   // context's stream); instance s is slot where[s].second of class where[s].first.
   std::vector<xpfc_ctx*> cls;
   std::vector<std::pair<int, int>> where;
+  std::vector<std::vector<int>> inst_of;  // class k, slot i -> instance number
   bool borrowed_stream = false;      // a class context: the stream belongs to its owner
   struct RagTables {                 // device class tables of one active-count configuration
     xpfc::RClass* d = nullptr;       // x (square / cubic boxes) | x (others) | y | z, back to back

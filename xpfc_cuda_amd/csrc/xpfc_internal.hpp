@@ -85,12 +85,17 @@ struct InstArgs {
 // indexes `slots`; instance `s` has its spectra `s * stride` complex elements
 // from the base pointers, its X partials `s * pstride` and its real-space
 // partials `s * pzstride` doubles from the partial base pointers.
+// In a budgeted block (xpfc_step_budget) the launches of step `sib` of the
+// block carry `budget`: the workgroups of an instance whose budget[slot] <= sib
+// return at once, so each instance takes its own number of the block's steps.
 struct Batch {
   const int* slots;
   const InstArgs* ia;
   long long stride;
   long long pstride;
   long long pzstride;
+  const int* budget;  // nullptr: every listed instance steps
+  int sib;
 };
 // Register-allocation variant flag (the arithmetic is identical), or-ed into
 // the real-space workgroup size: no occupancy hint.

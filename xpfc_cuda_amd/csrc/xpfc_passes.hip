@@ -127,7 +127,9 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
   using C = cpx<T>;
   __shared__ C lds[L * (TC + PADT)];
   __shared__ C twl[L];
-  buf += (long long)bt.slots[blockIdx.y] * bt.stride;
+  const int slot = bt.slots[blockIdx.y];
+  if (bt.budget && bt.sib >= bt.budget[slot]) return;  // the whole workgroup
+  buf += (long long)slot * bt.stride;
   strided_body<T, L, TC, DIR, NT>(buf, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), tw, lds, twl);
 }
 
@@ -292,6 +294,7 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
   __shared__ C lds[L * (TC + PADT)];
   __shared__ C twl[L];
   const int slot = bt.slots[blockIdx.y];
+  if (bt.budget && bt.sib >= bt.budget[slot]) return;
   work += (long long)slot * bt.stride;
   psi += (long long)slot * bt.stride;
   partial += (long long)slot * bt.pstride;
@@ -526,6 +529,7 @@ void k_zpass(const void* __restrict__ in, const cpx<T>* __restrict__ in_ny, void
   using C = cpx<T>;
   __shared__ C lds[z_lds_elems<T, H, NTZ>()];
   const int slot = bt.slots[blockIdx.y];
+  if (bt.budget && bt.sib >= bt.budget[slot]) return;
   const long long so = (long long)slot * bt.stride;  // complex elements
   zpass_body<T, H, NTZ, MODE, SAMPLE, PW>(reinterpret_cast<const C*>(in) + so, in_ny ? in_ny + so : nullptr,
                                           reinterpret_cast<C*>(out) + so, out_ny ? out_ny + so : nullptr,

@@ -1,44 +1,46 @@
-"""Continuous-batching service for XPFC simulation jobs.
+"""Continuous-batching service for XPFC simulation jobs (Python binding).
 
-A job is (grid, sigma, n0, steps, seed).  Jobs of one grid and precision share a
-size-class context (``BatchContext``) with a fixed number of instance slots; a
-job is admitted when its class has a free slot and its device footprint
-(``xpfc_instance_bytes``) fits the HBM budget beside the jobs already resident.
-Its field stays on the device from admission to completion.  Every size class
-advances its running jobs together, in blocks of up to 50 steps (captured
-hipGraphs inside the library), or up to `lookahead` such blocks at once while
-no queued job waits for the class, on its own stream, so the classes' work
-overlaps on the GPU; the host polls the classes without blocking and hands
-each one its next block as soon as its previous block has finished, admitting
-queued jobs into freed slots at that moment.  A job's result (final free energy, mean,
-optionally the field) is what the same job gives alone: the instances of a
-context never exchange data.
+A job is (grid, sigma, n0, steps, seed).  The scheduling runs in the library
+(``xpfc_service_*`` in the C ABI, xpfc_sched.cpp / xpfc_service.cpp): jobs are
+admitted in arrival order into free slots of their grid class while their
+device footprint (``xpfc_instance_bytes``) fits the HBM budget beside the
+resident jobs; the running jobs of the classes of one lane advance together
+in blocks of up to 50 steps, each block one captured graph with one launch
+per pass over the lane's grids (a context of several grids), each job by
+min(block, its remaining steps); lanes run on their own streams with their
+own block cadence (by default every grid class is a lane of its own, so a
+small grid's jobs are not held to a large grid's blocks); a job whose field
+diverges stops at the end of its block; finished
+jobs free their slot for the next admission.  Fields stay on the device from
+admission to completion.  A job's result (final free energy, mean, kept field)
+is what the same job gives alone.  This module builds the job requests and the
+arrival stream, submits jobs at their arrival times and polls the library.
 
 The reference runs one simulation per process (main.cu:20-435) and has no
-service; this module has no counterpart there.  It only calls the C ABI.
+service; this module has no counterpart there.
 """
 from __future__ import annotations
 
-import heapq
+import ctypes as ct
 import math
 import time
 from collections import deque
 from dataclasses import dataclass
-from typing import Callable, Dict, List, Optional, Sequence, Tuple
+from typing import Callable, Dict, Iterable, List, Optional, Sequence, Tuple
 
 import numpy as np
 
-from .api import (BatchContext, Params, XPFCError, XPFC_DCF_MAX, XPFC_ERR_NONFINITE,
-                  XPFC_FP32_MIXED, instance_bytes)
+from .api import (Params, XPFCError, XPFC_DCF_MAX, XPFC_FP32_MIXED, _Job, _JobResult,
+                  _ServiceClass, _ServiceConfig, _ServiceStats, _check, instance_bytes, lib)
 
 TWO_PI = 2.0 * math.pi
 BLOCK = 50      # steps per scheduling block (the library's graph block)
-# Blocks handed at once to a class that no queued job waits for.  Measured on
-# the 20 jobs/s stream (60 s windows): 1 -> p50 0.71 s, p99 4.7 s; 2 -> 0.47 s,
-# 6.2 s; 4 -> 0.29 s, 7.7 s; 8 -> 0.21 s, 9.7 s.  Longer runs of the small
-# classes take GPU share from the largest class, whose queue sets the tail, so
-# the default keeps the tail.
+# Blocks handed out at once while no queued job waits.  Measured on the 20
+# jobs/s stream with one context per size class: more lookahead shortened the
+# median and lengthened the tail, so the default keeps the tail.
 LOOKAHEAD = 1
+JOB_DONE, JOB_NONFINITE, JOB_REJECTED = 0, 1, 2
+_STATUS = {JOB_DONE: "done", JOB_NONFINITE: "nonfinite", JOB_REJECTED: "rejected"}
 
 
 def square_peaks() -> Tuple[Tuple[float, float, float], ...]:
@@ -66,7 +68,8 @@ class Job:
     F_ref: float = float("nan")
     f_mean: float = float("nan")
     mean: float = float("nan")
-    status: str = "queued"      # queued, running, done, nonfinite
+    status: str = "queued"      # queued, done, nonfinite, rejected
+    steps_done: int = 0
     field: Optional[np.ndarray] = None
 
     @property
@@ -100,58 +103,66 @@ def class_slots(cells_per_class: int = 1 << 25) -> Callable[[int, int], int]:
 default_slots = class_slots()
 
 
-class SizeClass:
-    """The context of one (dim, side) class and the jobs in its slots."""
-
-    def __init__(self, dim: int, side: int, nslots: int, template: Job, precision: int):
-        self.key = (dim, side)
-        self.nslots = nslots
-        self.precision = precision
-        p = job_params(template, precision)
-        self.ctx = BatchContext([p] * nslots)
-        self.ctx.set_active([])
-        self.free = list(range(nslots - 1, -1, -1))
-        self.running: Dict[int, Job] = {}      # slot -> job
-        self.remaining: Dict[int, int] = {}    # slot -> steps left
-        self.busy = False
-        self.active_key: Tuple[int, ...] = ()
-        self.cell_updates = 0
-
-    def close(self):
-        self.ctx.close()
+def lane_groups(spec: str):
+    """'class', 'one', or ';'-separated groups of ','-separated classes
+    ('2d_128,2d_256;3d_256'): the lanes argument of Service.  Classes of no
+    group get a lane each."""
+    if spec in ("class", "one"):
+        return spec
+    lane = {}
+    for i, grp in enumerate(spec.split(";")):
+        for name in grp.split(","):
+            name = name.strip()
+            if name:
+                d, side = name.split("d_")
+                lane[(int(d), int(side))] = i
+    nxt = len(spec.split(";"))
+    return lambda dim, side: lane.get((dim, side), nxt + 1000 * dim + side)
 
 
 class Service:
-    """Admission, slots and the polling loop.
+    """The library's job service over the grid classes of `classes` (or of
+    the jobs passed to run()/warm()).
 
     budget_bytes: device memory the resident jobs may occupy (sum of
     ``instance_bytes`` of the admitted jobs).  slots: function (dim, side) ->
-    slots of that size class.  keep_fields: download each job's final field.
-    lookahead: blocks handed to a class at once while no queued job waits for
-    it (fewer host round trips; a diverged job is then stopped up to that many
-    blocks later).
+    slots of that class.  keep_fields: download each job's final field.
+    lookahead: blocks per launch while no job waits.  lanes: function (dim,
+    side) -> lane number (classes with the same number share blocks), or
+    "class" (a lane per class) or "one" (every class in one lane).
     """
 
     def __init__(self, budget_bytes: float = 200e9, precision: int = XPFC_FP32_MIXED,
-                 slots: Callable[[int, int], int] = default_slots, keep_fields: bool = False,
-                 lookahead: int = LOOKAHEAD):
+                 slots: Callable[[int, int], int] = None, keep_fields: bool = False,
+                 lookahead: int = LOOKAHEAD, classes: Optional[Iterable[Tuple[int, int]]] = None,
+                 dx: float = 0.125, block: int = BLOCK, lanes="class"):
         if lookahead < 1:
             raise ValueError("lookahead must be >= 1")
         self.lookahead = int(lookahead)
+        self.block = int(block)
         self.budget = float(budget_bytes)
         self.precision = precision
-        self.slots_of = slots
+        self.slots_of = slots or default_slots
+        if lanes == "class":
+            self.lane_of = None
+        elif lanes == "one":
+            self.lane_of = lambda dim, side: 0
+        elif callable(lanes):
+            self.lane_of = lanes
+        else:
+            raise ValueError("lanes must be 'class', 'one' or a function (dim, side) -> int")
         self.keep_fields = keep_fields
-        self.classes: Dict[Tuple[int, int], SizeClass] = {}
-        self.queues: Dict[Tuple[int, int], deque] = {}  # per size class, arrival order
-        self.queued = 0
-        self.resident = 0
-        self.peak_resident = 0
+        self.dx = dx
+        self.keys: List[Tuple[int, int]] = sorted(set(classes or []))
+        self.handle = None
         self.done: List[Job] = []
         self.queue_samples: List[int] = []
-        self._fp: Dict[Tuple[int, int], int] = {}
         self.t0 = 0.0
+        self._jobs: Dict[int, Job] = {}
+        self._final: dict = {k: 0 for k, _ in _ServiceStats._fields_}
+        self._fp: Dict[Tuple[int, int], int] = {}
 
+    # -- set-up ---------------------------------------------------------------
     def footprint(self, job: Job) -> int:
         k = (job.dim, job.side)
         if k not in self._fp:
@@ -161,184 +172,188 @@ class Service:
             self._fp[k] = b
         return self._fp[k]
 
-    def now(self) -> float:
-        return time.perf_counter() - self.t0
+    def warm(self, keys: Sequence[Tuple[int, int]] = ()):
+        """Create the service (and its device context) for the given (dim,
+        side) classes and any declared before; jobs of other grids are then
+        rejected."""
+        if self.handle is not None:
+            return
+        self.keys = sorted(set(self.keys) | set(keys))
+        if not self.keys:
+            raise ValueError("no grid classes")
+        arr = (_ServiceClass * len(self.keys))()
+        for i, (dim, side) in enumerate(self.keys):
+            arr[i].dim = dim
+            for d in range(3):
+                arr[i].n[d] = side if d < dim else 1
+                arr[i].L[d] = side * self.dx if d < dim else 1.0
+            arr[i].slots = int(self.slots_of(dim, side))
+            arr[i].lane = i if self.lane_of is None else int(self.lane_of(dim, side))
+        cfg = _ServiceConfig(precision=self.precision, device=0, budget_bytes=self.budget,
+                             block_steps=self.block, lookahead=self.lookahead,
+                             keep_fields=1 if self.keep_fields else 0, nclasses=len(self.keys),
+                             classes=ct.cast(arr, ct.POINTER(_ServiceClass)))
+        h = ct.c_void_p()
+        _check(lib().xpfc_service_create(ct.byref(cfg), ct.byref(h)), "xpfc_service_create")
+        self.handle = h
+        self.t0 = time.perf_counter()
 
-    def submit(self, job: Job):
+    @property
+    def classes(self) -> Dict[str, int]:
+        return {"%dd_%d" % k: int(self.slots_of(*k)) for k in self.keys}
+
+    # -- requests -------------------------------------------------------------
+    def _request(self, job: Job, arrival: float) -> _Job:
+        r = _Job()
+        r.id = job.jid
+        r.params = job_params(job, self.precision).to_c()
+        r.steps = job.steps
+        r.amplitude = job.amplitude
+        r.seed = job.seed
+        r.arrival = arrival
+        return r
+
+    def submit(self, job: Job, arrival: float = -1.0):
+        """Queue a job now (arrival < 0) or at the given service time; raises
+        ValueError for a job that can never run (unknown grid class, steps < 1,
+        footprint beyond the budget)."""
         if job.steps < 1:
             raise ValueError("job %d: steps must be >= 1" % job.jid)
-        fp = self.footprint(job)  # rejects unsupported grids at submission
-        if fp > self.budget:
+        if self.footprint(job) > self.budget:
             raise ValueError("job %d: device footprint %d bytes exceeds the budget of %d bytes"
-                             % (job.jid, fp, int(self.budget)))
-        self.queues.setdefault((job.dim, job.side), deque()).append(job)
-        self.queued += 1
-
-    def warm(self, keys: Sequence[Tuple[int, int]]):
-        """Create the size-class contexts and tune their passes ahead of the
-        first job (otherwise the first job of a class pays for both)."""
-        for dim, side in keys:
-            c = self._class(Job(jid=-1, dim=dim, side=side, sigma=0.1, n0=0.05, steps=1, seed=0))
-            c.ctx.autotune()
-
-    # -- per class -------------------------------------------------------------
-    def _class(self, job: Job) -> SizeClass:
-        k = (job.dim, job.side)
-        c = self.classes.get(k)
-        if c is None:
-            c = SizeClass(job.dim, job.side, self.slots_of(job.dim, job.side), job, self.precision)
-            self.classes[k] = c
-        return c
-
-    def _complete(self, c: SizeClass):
-        finished = [s for s, r in c.remaining.items() if r == 0]
-        for slot in finished:
-            job = c.running.pop(slot)
-            del c.remaining[slot]
-            if job.status != "nonfinite":
-                try:
-                    job.F_ref, job.f_mean = c.ctx.fe_calc(slot)
-                    job.status = "done"
-                except XPFCError as e:
-                    if e.status != XPFC_ERR_NONFINITE:
-                        raise
-                    job.status = "nonfinite"
-            job.mean = c.ctx.avg_calc(slot)
-            if self.keep_fields:
-                job.field = c.ctx.get_field(slot)
-            job.finished = self.now()
-            c.free.append(slot)
-            self.resident -= self.footprint(job)
-            self.done.append(job)
-        if finished:  # a freed slot leaves the active list before it is reused
-            c.active_key = tuple(sorted(c.running))
-            c.ctx.set_active(list(c.active_key))
-
-    def _admit(self):
-        """Admit queued jobs, in arrival order over all classes, into idle
-        classes with free slots.  Memory is granted in arrival order: once a
-        job waits for memory, no later job is admitted before it."""
-        heap = [(q[0].arrival, k) for k, q in self.queues.items() if q]
-        heapq.heapify(heap)
-        while heap:
-            _, k = heapq.heappop(heap)
-            q = self.queues[k]
-            job = q[0]
-            c = self._class(job)
-            if c.busy or not c.free:
-                continue  # this class waits for its block or a slot
-            fp = self.footprint(job)
-            if self.resident + fp > self.budget:
-                return
-            q.popleft()
-            self.queued -= 1
-            self._start(c, job, fp)
-            if q:
-                heapq.heappush(heap, (q[0].arrival, k))
-
-    def _start(self, c: SizeClass, job: Job, fp: int):
-        slot = c.free.pop()
-        c.ctx.set_params(slot, job_params(job, self.precision))
-        c.ctx.reset(slot)
-        c.ctx.initialize_noise(slot, job.amplitude, job.seed)
-        c.running[slot] = job
-        c.remaining[slot] = job.steps
-        job.status = "running"
-        job.admitted = self.now()
-        self.resident += fp
-        self.peak_resident = max(self.peak_resident, self.resident)
-
-    def _launch(self, c: SizeClass):
-        if not c.running:
-            return
-        slots = tuple(sorted(c.running))
-        if slots != c.active_key:
-            c.ctx.set_active(list(slots))
-            c.active_key = slots
-        cap = BLOCK if self.queues.get(c.key) else BLOCK * self.lookahead
-        n = min(cap, min(c.remaining.values()))
-        c.ctx.step(n)
-        for s in slots:
-            c.remaining[s] -= n
-        c.cell_updates += n * len(slots) * c.running[slots[0]].cells
-        c.busy = True
+                             % (job.jid, self.footprint(job), int(self.budget)))
+        if self.handle is None:
+            self.warm([(job.dim, job.side)])
+        r = self._request(job, arrival)
+        st = lib().xpfc_service_submit(self.handle, ct.byref(r))
+        if st != 0:
+            self._collect()
+            raise ValueError("job %d rejected by the service (%s)" % (job.jid, lib().xpfc_last_error().decode()))
+        self._jobs[job.jid] = job
+        job.status = "queued"
 
     def poll(self) -> bool:
-        """One pass over the classes; True if any work was handed out."""
-        worked = False
-        for c in list(self.classes.values()):
-            if c.busy:
-                if not c.ctx.query():
-                    continue
-                c.busy = False
-                # the explicit nonlinear term diverges for strongly
-                # crystallising parameters: such a job stops at the end of
-                # the block it diverged in and its slot is reused
-                ok = c.ctx.probe_finite()
-                for slot, job in c.running.items():
-                    if not ok[slot]:
-                        job.status = "nonfinite"
-                        c.remaining[slot] = 0
-                self._complete(c)
-                worked = True
-        self._admit()
-        for c in self.classes.values():
-            if not c.busy and c.running:
-                self._launch(c)
-                worked = True
-        return worked
+        r = lib().xpfc_service_poll(self.handle)
+        if r < 0:
+            _check(-r, "xpfc_service_poll")
+        self._collect()
+        return r > 0
 
+    def idle(self) -> bool:
+        return bool(lib().xpfc_service_idle(self.handle))
+
+    def _collect(self):
+        buf = (_JobResult * 64)()
+        while True:
+            n = lib().xpfc_service_results(self.handle, buf, 64)
+            if n < 0:
+                _check(-n, "xpfc_service_results")
+            for r in buf[:n]:
+                job = self._jobs.pop(int(r.id), None)
+                if job is None:
+                    continue
+                job.status = _STATUS[r.status]
+                job.F_ref, job.f_mean, job.mean = r.F_ref, r.f_mean, r.mean
+                job.steps_done = int(r.steps_done)
+                job.admitted, job.finished = r.t_admitted, r.t_finished
+                job.arrival = r.t_arrival
+                if self.keep_fields and r.status != JOB_REJECTED:
+                    f = np.empty(job.cells, dtype=np.float64)
+                    got = lib().xpfc_service_field(self.handle, job.jid, f.ctypes.data_as(ct.POINTER(ct.c_double)),
+                                                   job.cells)
+                    if got < 0:
+                        _check(int(-got), "xpfc_service_field")
+                    job.field = f.reshape(job.shape)
+                self.done.append(job)
+            if n < 64:
+                break
+
+    def stats(self) -> dict:
+        if self.handle is None:  # closed: the figures it ended with
+            return dict(self._final)
+        st = _ServiceStats()
+        _check(lib().xpfc_service_get_stats(self.handle, ct.byref(st)), "xpfc_service_get_stats")
+        return {k: getattr(st, k) for k, _ in _ServiceStats._fields_}
+
+    @property
+    def peak_resident(self) -> float:
+        return self.stats()["peak_resident_bytes"]
+
+    # -- the serving loop -----------------------------------------------------
     def run(self, jobs: Sequence[Job], realtime: bool = True, idle_sleep: float = 1e-4,
             progress: Optional[Callable[[str], None]] = None) -> dict:
-        """Serve `jobs` (sorted by arrival).  realtime: submit each job at its
-        arrival time; otherwise submit all at once.  progress: called with a
-        status line every 10 s."""
+        """Serve `jobs` (by arrival): realtime submits each job at its arrival
+        time on the service clock, otherwise all at once.  The library admits,
+        runs and finishes them; this loop only submits and polls.  progress:
+        called with a status line every 10 s."""
+        self.warm(sorted({(j.dim, j.side) for j in jobs}))
         pending = deque(sorted(jobs, key=lambda j: j.arrival))
-        self.t0 = time.perf_counter()
         total = len(pending)
-        last_q = -1.0
-        last_p = 0.0
-        while len(self.done) < total:
-            t = self.now()
+        start = len(self.done)
+        wall0 = time.perf_counter()
+        last_q = last_p = -1e9
+        while len(self.done) - start < total:
+            t = time.perf_counter() - self.t0
+            while pending and (not realtime or pending[0].arrival <= time.perf_counter() - wall0):
+                job = pending.popleft()
+                # on the service clock: the scheduled arrival, or now
+                self.submit(job, arrival=job.arrival + (wall0 - self.t0) if realtime else -1.0)
+            if t - last_q >= 0.1:
+                self.queue_samples.append(int(self.stats()["queued"]))
+                last_q = t
             if progress and t - last_p >= 10.0:
                 last_p = t
+                s = self.stats()
                 progress("t=%.0fs done=%d queued=%d running=%d resident=%.1fGB"
-                         % (t, len(self.done), self.queued,
-                            sum(len(c.running) for c in self.classes.values()), self.resident / 1e9))
-            while pending and (not realtime or pending[0].arrival <= t):
-                self.submit(pending.popleft())
-            if t - last_q >= 0.1:
-                self.queue_samples.append(self.queued)
-                last_q = t
+                         % (t, len(self.done) - start, s["queued"], s["running"], s["resident_bytes"] / 1e9))
             if not self.poll():
                 if pending and realtime:
-                    time.sleep(min(idle_sleep, max(0.0, pending[0].arrival - self.now())))
+                    time.sleep(min(idle_sleep, max(0.0, pending[0].arrival - (time.perf_counter() - wall0))))
                 else:
                     time.sleep(idle_sleep)
-        wall = self.now()
-        return self.report(wall)
+        return self.report(time.perf_counter() - wall0, self.done[start:])
 
-    def report(self, wall: float) -> dict:
-        lat = np.array([j.latency for j in self.done])
-        cu = sum(c.cell_updates for c in self.classes.values())
+    def report(self, wall: float, done: Optional[List[Job]] = None) -> dict:
+        done = self.done if done is None else done
+        ran = [j for j in done if j.status != "rejected"]
+        lat = np.array([j.latency for j in ran])
+        s = self.stats()
         return {
-            "jobs": len(self.done),
-            "nonfinite_jobs": sum(j.status == "nonfinite" for j in self.done),
+            "jobs": len(done),
+            "nonfinite_jobs": sum(j.status == "nonfinite" for j in done),
+            "rejected_jobs": sum(j.status == "rejected" for j in done),
             "wall_s": wall,
             "p50_s": float(np.percentile(lat, 50)) if lat.size else float("nan"),
             "p99_s": float(np.percentile(lat, 99)) if lat.size else float("nan"),
             "mean_s": float(lat.mean()) if lat.size else float("nan"),
-            "cell_updates_per_s": cu / wall if wall > 0 else 0.0,
-            "queue_max": int(max(self.queue_samples)) if self.queue_samples else 0,
+            "cell_updates_per_s": s["cell_updates"] / wall if wall > 0 else 0.0,
+            "queue_max": int(s["queue_max"]),
             "queue_mean": float(np.mean(self.queue_samples)) if self.queue_samples else 0.0,
-            "peak_resident_gb": self.peak_resident / 1e9,
-            "classes": {"%dd_%d" % k: c.nslots for k, c in sorted(self.classes.items())},
+            "peak_resident_gb": s["peak_resident_bytes"] / 1e9,
+            "blocks": int(s["blocks"]),
+            "classes": self.classes,
+            "by_class": self._by_class(ran),
         }
 
+    @staticmethod
+    def _by_class(ran: List[Job]) -> dict:
+        """Per grid class: jobs, median / 99th-percentile latency and the
+        median wait for admission, in seconds."""
+        out = {}
+        for key in sorted({(j.dim, j.side) for j in ran}):
+            js = [j for j in ran if (j.dim, j.side) == key]
+            lat = np.array([j.latency for j in js])
+            wait = np.array([j.admitted - j.arrival for j in js])
+            out["%dd_%d" % key] = {"jobs": len(js), "p50_s": round(float(np.percentile(lat, 50)), 4),
+                                   "p99_s": round(float(np.percentile(lat, 99)), 4),
+                                   "wait_p50_s": round(float(np.percentile(wait, 50)), 4)}
+        return out
+
     def close(self):
-        for c in self.classes.values():
-            c.close()
-        self.classes.clear()
+        if self.handle is not None:
+            self._final = self.stats()
+            lib().xpfc_service_destroy(self.handle)
+            self.handle = None
 
 
 def batch_jobs(ninst: int = 64, steps: int = 2000, seed: int = 0,
