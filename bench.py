@@ -244,14 +244,15 @@ def batched_run(steps=2000, ninst=64, seed=0):
 
 
 def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 << 25,
-                lookahead=1, lanes="class"):
+                lookahead=1, lanes="class", rounding="smooth"):
     """Poisson job stream through the continuous-batching service."""
     from xpfc_cuda_amd import serving as S
-    jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d)
+    jobs = S.poisson_jobs(rate, seconds, seed=seed, frac_3d=frac_3d, rounding=rounding)
     svc = S.Service(budget_bytes=budget, slots=S.class_slots(class_cells), lookahead=lookahead,
                     lanes=S.lane_groups(lanes))
     try:
-        svc.warm([(2, 128), (2, 256), (2, 512), (2, 1024), (3, 64), (3, 128), (3, 256)])
+        svc.warm([(2, v) for v in S.grid_sides(128, 1024, rounding)] +
+                 [(3, v) for v in S.grid_sides(64, 256, rounding)])
         rep = svc.run(jobs, realtime=True,
                       progress=lambda m: print("serving " + m, file=sys.stderr, flush=True))
         # offered load: cell-updates per second the arrivals ask for
@@ -259,7 +260,9 @@ def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 
         rep.update({"arrival_rate": rate, "window_s": seconds, "frac_3d": frac_3d,
                     "offered_cell_updates_per_s": offered,
                     "budget_gb": budget / 1e9, "lookahead_blocks": lookahead, "lanes": lanes,
-                    "rounding": "side log-uniform, rounded to the nearest power of two in log2",
+                    "rounding": ("side log-uniform, rounded to the nearest 2^k, 3*2^k or 5*2^k in log2"
+                                 if rounding == "smooth" else
+                                 "side log-uniform, rounded to the nearest power of two in log2"),
                     "steps": "U{500..5000}", "sigma": "U[0.05,0.2]", "n0": "U[0.02,0.08]"})
     finally:
         svc.close()
@@ -289,6 +292,8 @@ def main():
                     help="'class' (a lane, with its own stream and block cadence, per size class), "
                          "'one' (one lane for all), or groups of classes such as "
                          "'2d_128,2d_256;2d_512,2d_1024;3d_64,3d_128;3d_256'")
+    ap.add_argument("--serve-rounding", choices=["smooth", "pow2"], default="smooth",
+                    help="job sides rounded to 2^k, 3*2^k, 5*2^k (smooth) or to powers of two")
     ap.add_argument("--only", choices=["batch", "serve", "configs"],
                     help="run only that leg and print its JSON")
     ap.add_argument("--traffic-probe", action="store_true",
@@ -326,7 +331,8 @@ def main():
         print(json.dumps({"serving": serving_run(args.serve, args.serve_rate, args.serve_3d,
                                                   class_cells=args.serve_class_cells,
                                                   lookahead=args.serve_lookahead,
-                                                  lanes=args.serve_lanes)}), flush=True)
+                                                  lanes=args.serve_lanes,
+                                                  rounding=args.serve_rounding)}), flush=True)
         return
     n = args.n
     pin = [int(v) for v in args.variants.split(",")] if args.variants else None
@@ -397,7 +403,8 @@ def main():
     if args.serve > 0:
         out["serving"] = serving_run(args.serve, args.serve_rate, args.serve_3d,
                                      class_cells=args.serve_class_cells,
-                                     lookahead=args.serve_lookahead, lanes=args.serve_lanes)
+                                     lookahead=args.serve_lookahead, lanes=args.serve_lanes,
+                                     rounding=args.serve_rounding)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(n)
     print(json.dumps(out), flush=True)

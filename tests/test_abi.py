@@ -188,8 +188,11 @@ def test_read_input_restart_box_from_vtk(tmp_path, monkeypatch):
 
 
 @pytest.mark.parametrize("n,L,msg", [
-    ((48, 32), (6.0, 4.0), "power of two"),      # not a power of two
-    ((4, 32), (0.5, 4.0), "power of two"),       # below the smallest line
+    ((56, 32), (7.0, 4.0), "3*2^k"),             # 7 * 8: no compiled line length
+    ((36, 32), (4.5, 4.0), "3*2^k"),             # 9 * 4: a factor 3 twice
+    ((10, 32), (1.25, 4.0), "3*2^k"),            # 5 * 2: its half line (5) is not compiled
+    ((4, 32), (0.5, 4.0), "3*2^k"),              # below the smallest line
+    ((4096, 32), (512.0, 4.0), "3*2^k"),         # above the longest line
     ((32, 32), (0.0, 4.0), "box length"),        # empty box
 ])
 def test_create_rejects_bad_grids_before_touching_a_device(n, L, msg):
@@ -206,7 +209,11 @@ def test_instance_bytes_counts_three_spectra():
     b32, b64 = X.instance_bytes(p32), X.instance_bytes(p64)
     assert 3 * spectrum32 <= b32 < 3 * spectrum32 + (1 << 20)
     assert 3 * 2 * spectrum32 <= b64 < 3 * 2 * spectrum32 + (1 << 20)
-    assert X.instance_bytes(X.Params(n=(48, 48), L=(6.0, 6.0))) == 0
+    assert X.instance_bytes(X.Params(n=(56, 56), L=(7.0, 7.0))) == 0
+    # 3*2^k and 5*2^k extents are sized like any other
+    spectrum48 = 48 * 25 * 16
+    assert 3 * spectrum48 <= X.instance_bytes(X.Params(n=(48, 48), L=(6.0, 6.0))) < 3 * spectrum48 + (1 << 20)
+    assert X.instance_bytes(X.Params(n=(2048, 2048), L=(256.0, 256.0))) >= 3 * 2048 * 1025 * 16
 
 
 def test_read_vtk_rejects_short_or_garbled_data(tmp_path):

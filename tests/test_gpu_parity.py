@@ -252,17 +252,18 @@ def test_mass_conserved_fp32():
         assert math.isfinite(F)
 
 
-@pytest.mark.parametrize("shape", [(64, 64), (32, 64, 32), (16, 128, 128)])
+@pytest.mark.parametrize("shape", [(64, 64), (32, 64, 32), (16, 128, 128), (256, 16, 16)])
 @pytest.mark.parametrize("prec", PRECS)
 def test_variants_give_bitwise_identical_fields(shape, prec):
     """Tile width, workgroup order and block size only regroup lines or
     reorder workgroups: the field must not change.  (16, 128, 128) has enough
-    axis-0 tiles for the run-dealt XCD order."""
+    axis-0 tiles for the run-dealt XCD order; (256, 16, 16) has long axis-0
+    lines with 2-column tiles available only where 4 do not fit."""
     op, lp = growth_case(shape, prec=prec, t_sample=0)
     psi0 = O.noise_ic(op, 0.01, 9)
     outs = []
     for v in [(4, 4, 64), (8, 8, 128), (16, 16, 256), (0, 0, 512), (16, 0, 257), (8, 0, 65),
-              (4, 5, 64), (16, 17, 257)]:
+              (4, 5, 64), (16, 17, 257), (2, 3, 64)]:
         with Context(lp) as ctx:
             try:
                 ctx.set_variants(*v)
@@ -281,7 +282,7 @@ def test_autotune_picks_available_variant():
     with Context(lp) as ctx:
         v = ctx.autotune()
         # v[1] may carry the XCD-pairs bit, v[2] the no-occupancy-hint bit
-        assert v[0] in (4, 8, 16) and (v[1] & ~1) in (4, 8, 16) and (v[2] & ~1) in (64, 128, 256, 512)
+        assert v[0] in (2, 4, 8, 16) and (v[1] & ~1) in (2, 4, 8, 16) and (v[2] & ~1) in (64, 128, 256, 512)
         assert ctx.variants() == v
 
 

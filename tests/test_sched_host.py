@@ -285,10 +285,17 @@ def test_poisson_stream_statistics():
     assert all(j.seed == j.jid for j in jobs)
     d2 = [j.side for j in jobs if j.dim == 2]
     d3 = [j.side for j in jobs if j.dim == 3]
-    assert set(d2) <= {128, 256, 512, 1024} and set(d3) <= {64, 128, 256}
-    # nearest power of two in log2: the end points get half the mass of the inner sides
+    assert set(d2) == {128, 160, 192, 256, 320, 384, 512, 640, 768, 1024}
+    assert set(d3) == {64, 80, 96, 128, 160, 192, 256}
+    # nearest side in log2: a side takes the log-mass between the midpoints to
+    # its neighbours (128 in [64, 256]: from log2(sqrt(96*128)) to log2(sqrt(128*160)))
     f = np.mean(np.array(d3) == 128)
-    assert f == pytest.approx(0.5, abs=0.04)
+    want = (0.5 * math.log2(160 / 96)) / 2.0
+    assert f == pytest.approx(want, abs=0.03)
+    # the power-of-two rounding of the first build stays available
+    p2 = S.poisson_jobs(rate=20.0, duration=50.0, seed=3, rounding="pow2")
+    assert {j.side for j in p2 if j.dim == 3} == {64, 128, 256}
+    assert np.mean([j.side == 128 for j in p2 if j.dim == 3]) == pytest.approx(0.5, abs=0.08)
     assert all(500 <= j.steps <= 5000 for j in jobs)
     assert all(0.05 <= j.sigma <= 0.2 and 0.02 <= j.n0 <= 0.08 for j in jobs)
 

@@ -27,8 +27,11 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contra
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include")]
 
-SOURCES = ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp", "xpfc_sched.cpp",
-           "xpfc_service.cpp"]
+# the step passes of each line length compile in their own units (xpfc_len_*.hip), in parallel
+LEN_UNITS = ["xpfc_len_p2a.hip", "xpfc_len_p2b.hip", "xpfc_len_p2c.hip", "xpfc_len_p2d.hip",
+             "xpfc_len_r3a.hip", "xpfc_len_r3b.hip", "xpfc_len_r5a.hip", "xpfc_len_r5b.hip"]
+SOURCES = LEN_UNITS + ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp",
+                       "xpfc_sched.cpp", "xpfc_service.cpp"]
 
 
 def _hipcc() -> str:

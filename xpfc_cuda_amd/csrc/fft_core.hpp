@@ -108,21 +108,159 @@ template <int DIR, typename C> struct Dft<16, DIR, C> {
   }
 };
 
-// ---- stage schedule: greedy radix E, remainder last ------------------------
+// ---- radices with a factor 3 or 5 ------------------------------------------
+// cos and sin of d degrees for the angles of the radix-3/5/6/10/12/20
+// butterflies (multiples of 18 and 30 degrees), as exact decimal constants.
+struct CS {
+  double c, s;
+};
+__host__ __device__ constexpr CS cs_first_quadrant(int d) {
+  return d == 0    ? CS{1.0, 0.0}
+         : d == 18 ? CS{0.95105651629515357211643933, 0.30901699437494742410229341}
+         : d == 30 ? CS{0.86602540378443864676372317, 0.5}
+         : d == 36 ? CS{0.80901699437494742410229341, 0.58778525229247312916870595}
+         : d == 54 ? CS{0.58778525229247312916870595, 0.80901699437494742410229341}
+         : d == 60 ? CS{0.5, 0.86602540378443864676372317}
+         : d == 72 ? CS{0.30901699437494742410229341, 0.95105651629515357211643933}
+         : d == 90 ? CS{0.0, 1.0}
+                   : CS{0.0, 0.0};  // not an angle of these radices
+}
+__host__ __device__ constexpr CS cs_deg(int d) {
+  d %= 360;
+  return d <= 90    ? cs_first_quadrant(d)
+         : d <= 180 ? CS{-cs_first_quadrant(180 - d).c, cs_first_quadrant(180 - d).s}
+         : d <= 270 ? CS{-cs_first_quadrant(d - 180).c, -cs_first_quadrant(d - 180).s}
+                    : CS{cs_first_quadrant(360 - d).c, -cs_first_quadrant(360 - d).s};
+}
+
+// a * exp(DIR * 2*pi*i * m / R) with compile-time m, R (R | 360)
+template <int DIR, int R, int M, typename C> __device__ __forceinline__ C twiddle_const(C a) {
+  using T = decltype(C::x);
+  constexpr int d = (360 / R) * (M % R);
+  if constexpr (d == 0) return a;
+  else if constexpr (d == 90) return rot4<DIR>(a);
+  else if constexpr (d == 180) return mk<C>(-a.x, -a.y);
+  else if constexpr (d == 270) return rot4<-DIR>(a);
+  else {
+    constexpr CS w = cs_deg(d);
+    return cmul(a, mk<C>((T)w.c, (T)(DIR * w.s)));
+  }
+}
+
+template <int DIR, typename C> struct Dft<3, DIR, C> {
+  static __device__ __forceinline__ void run(C* x) {
+    using T = decltype(C::x);
+    const T s = (T)(DIR * 0.86602540378443864676372317);  // DIR * sin(2 pi / 3)
+    const C t = cadd(x[1], x[2]);
+    const C d = csub(x[1], x[2]);
+    const C m = mk<C>(x[0].x - (T)0.5 * t.x, x[0].y - (T)0.5 * t.y);  // cos(2 pi / 3) = -1/2
+    const C r = mk<C>(-s * d.y, s * d.x);                               // DIR i sin(2 pi / 3) (x1 - x2)
+    x[0] = cadd(x[0], t);
+    x[1] = cadd(m, r);
+    x[2] = csub(m, r);
+  }
+};
+template <int DIR, typename C> struct Dft<5, DIR, C> {
+  static __device__ __forceinline__ void run(C* x) {
+    using T = decltype(C::x);
+    const T c1 = (T)0.30901699437494742410229341, c2 = (T)-0.80901699437494742410229341;  // cos(2pi/5), cos(4pi/5)
+    const T s1 = (T)(DIR * 0.95105651629515357211643933), s2 = (T)(DIR * 0.58778525229247312916870595);
+    const C a1 = cadd(x[1], x[4]), b1 = csub(x[1], x[4]);
+    const C a2 = cadd(x[2], x[3]), b2 = csub(x[2], x[3]);
+    const C m1 = mk<C>(x[0].x + c1 * a1.x + c2 * a2.x, x[0].y + c1 * a1.y + c2 * a2.y);
+    const C m2 = mk<C>(x[0].x + c2 * a1.x + c1 * a2.x, x[0].y + c2 * a1.y + c1 * a2.y);
+    // i * (s1 b1 + s2 b2) and i * (s2 b1 - s1 b2)
+    const C r1 = mk<C>(-(s1 * b1.y + s2 * b2.y), s1 * b1.x + s2 * b2.x);
+    const C r2 = mk<C>(-(s2 * b1.y - s1 * b2.y), s2 * b1.x - s1 * b2.x);
+    x[0] = mk<C>(x[0].x + a1.x + a2.x, x[0].y + a1.y + a2.y);
+    x[1] = cadd(m1, r1);
+    x[4] = csub(m1, r1);
+    x[2] = cadd(m2, r2);
+    x[3] = csub(m2, r2);
+  }
+};
+
+// Composite R = P*Q (Cooley-Tukey, natural order in and out): Q-point DFTs of
+// the P decimated sequences x[n1 + P n2], twiddles W_R^(n1 k2), then P-point
+// DFTs giving X[k2 + Q k1].
+template <int P, int Q, int DIR, typename C> struct DftPQ {
+  template <int N1, int K2> static __device__ __forceinline__ void tw(C (&y)[P][Q]) {
+    if constexpr (N1 < P) {
+      if constexpr (K2 < Q) {
+        y[N1][K2] = twiddle_const<DIR, P * Q, N1 * K2>(y[N1][K2]);
+        tw<N1, K2 + 1>(y);
+      } else {
+        tw<N1 + 1, 0>(y);
+      }
+    }
+  }
+  static __device__ __forceinline__ void run(C* x) {
+    C y[P][Q];
+#pragma unroll
+    for (int n1 = 0; n1 < P; ++n1) {
+#pragma unroll
+      for (int n2 = 0; n2 < Q; ++n2) y[n1][n2] = x[n1 + P * n2];
+      Dft<Q, DIR, C>::run(y[n1]);
+    }
+    tw<1, 1>(y);
+#pragma unroll
+    for (int k2 = 0; k2 < Q; ++k2) {
+      C z[P];
+#pragma unroll
+      for (int n1 = 0; n1 < P; ++n1) z[n1] = y[n1][k2];
+      Dft<P, DIR, C>::run(z);
+#pragma unroll
+      for (int k1 = 0; k1 < P; ++k1) x[k2 + Q * k1] = z[k1];
+    }
+  }
+};
+template <int DIR, typename C> struct Dft<6, DIR, C> : DftPQ<2, 3, DIR, C> {};
+template <int DIR, typename C> struct Dft<10, DIR, C> : DftPQ<2, 5, DIR, C> {};
+template <int DIR, typename C> struct Dft<12, DIR, C> : DftPQ<4, 3, DIR, C> {};
+template <int DIR, typename C> struct Dft<20, DIR, C> : DftPQ<4, 5, DIR, C> {};
+
+// ---- stage schedule ----------------------------------------------------------
+// Each stage takes the largest radix that divides both E (a thread's points,
+// so a thread holds whole butterflies) and the length still to transform.
+// For power-of-two L and E this is radix E greedily with the remainder last.
+__host__ __device__ constexpr int pick_radix(int rem, int E) {
+  for (int r = E; r > 1; --r)
+    if (E % r == 0 && rem % r == 0) return r;
+  return rem;
+}
 __host__ __device__ constexpr int radix_at(int L, int E, int s) {
   int rem = L;
-  for (int i = 0; i < s; ++i) rem /= (rem >= E ? E : rem);
-  return rem >= E ? E : rem;
+  for (int i = 0; i < s; ++i) rem /= pick_radix(rem, E);
+  return pick_radix(rem, E);
 }
 __host__ __device__ constexpr int nstages(int L, int E) {
   int rem = L, n = 0;
-  while (rem > 1) { rem /= (rem >= E ? E : rem); ++n; }
+  while (rem > 1) { rem /= pick_radix(rem, E); ++n; }
   return n;
 }
 __host__ __device__ constexpr int ns_at(int L, int E, int s) {
   int ns = 1;
   for (int i = 0; i < s; ++i) ns *= radix_at(L, E, i);
   return ns;
+}
+
+// ---- points per thread -------------------------------------------------------
+// Strided lines of L points: 8 for 2^k (radix-8 stages), 12 for 3*2^k (a
+// radix-12 stage, then 4s and a 2), 20 for 5*2^k (radix 20, then 4s and a 2);
+// a shorter line is one thread's.
+template <int L> constexpr int elems_per_thread() {
+  return L % 5 == 0 ? (L < 20 ? L : 20) : L % 3 == 0 ? (L < 12 ? L : 12) : (L < 8 ? L : 8);
+}
+// Real-space half lines of H points: for 2^k, radix-16 FP32 lines from H = 128
+// up (one LDS exchange per transform at H = 128, 256) and 16 points per thread
+// at H = 1024 (the line then fills one wavefront); 12 points for 3*2^k, 20 for
+// 5*2^k.  Every choice leaves a power-of-two number of threads per line.
+template <typename T, int H> constexpr int z_elems() {
+  return H % 5 == 0   ? (H < 20 ? H : 20)
+         : H % 3 == 0 ? (H < 12 ? H : 12)
+         : H < 8      ? H
+         : (H >= 1024 || (H >= 128 && sizeof(T) == 4)) ? 16
+                                                        : 8;
 }
 
 // One Stockham stage computed in registers (no data movement).
@@ -137,7 +275,7 @@ __device__ __forceinline__ void stage_compute(cpx<T> (&v)[E], int t, const cpx<T
     for (int q = 0; q < R; ++q) x[q] = v[u + q * B];
     if constexpr (NS > 1) {
       const int j = t + u * (L / E);
-      const int jj = j & (NS - 1);
+      const int jj = j % NS;
 #pragma unroll
       for (int q = 1; q < R; ++q) {
         C w = tw[(q * jj) * (L / (NS * R))];
@@ -165,7 +303,7 @@ __device__ __forceinline__ void fft_stages(cpx<T> (&v)[E], int t, cpx<T>* lds, c
 #pragma unroll
     for (int u = 0; u < B; ++u) {
       const int j = t + u * (L / E);
-      const int jj = j & (NS - 1);
+      const int jj = j % NS;
       const int base = (j - jj) * R + jj;
 #pragma unroll
       for (int q = 0; q < R; ++q) lds[idx(base + q * NS)] = v[u + q * B];

@@ -49,7 +49,15 @@ constexpr int SUM_ROWS = 1024;  // sampled steps held on the device per read-bac
 constexpr int DIAG_ROW = SUM_ROWS;  // extra row for on-demand diagnostics
 constexpr int GRAPH_BLOCK = 50;  // steps per captured step block
 
-bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+// Grid extents the step passes compile: 8 .. 2048 points of the form f * 2^k
+// with f in {1, 3, 5} and k >= 2 (so the half length n/2 of the contiguous
+// axis is one of the compiled line lengths too).
+bool supported_side(int n) {
+  if (n < 8 || n > 2048) return false;
+  int f = n;
+  while (f % 2 == 0) f /= 2;
+  return (f == 1 || f == 3 || f == 5) && n % (4 * f) == 0;
+}
 
 // Plane stride (complex elements) of the packed main block: planes of 64 KiB
 // or more get 128 bytes of padding, so the axis-0 pass does not stride by a
@@ -150,10 +158,11 @@ int upload_twiddles(xpfc_ctx* c, void** dst, int L, int denom) {
   return XPFC_OK;
 }
 
-// widest valid tile (16 columns FP32 / 8 FP64 keeps every row access 128 bytes)
+// widest valid tile (16 columns FP32 / 8 FP64 keeps every row access 128 bytes;
+// FP64 lines of 2048 points fit only 2 columns in LDS)
 int default_tc(int prec, int L, const Segs& s) {
   const int pref = prec == 1 ? 16 : 8;
-  for (int tc = pref; tc >= 4; tc /= 2)
+  for (int tc = pref; tc >= 2; tc /= 2)
     if (strided_blocks(prec, L, tc, s) >= 0) return tc;
   return -1;
 }
@@ -327,8 +336,9 @@ int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out
   }
   const int nd = p.dim;
   for (int d = 0; d < nd; ++d) {
-    if (!is_pow2(p.n[d]) || p.n[d] < 8 || p.n[d] > 1024)
-      return fail(XPFC_ERR_ARG, "grid extent n[%d]=%d must be a power of two in [8, 1024]", d, p.n[d]);
+    if (!supported_side(p.n[d]))
+      return fail(XPFC_ERR_ARG, "grid extent n[%d]=%d must be 2^k, 3*2^k or 5*2^k points in [8, 2048]", d,
+                  p.n[d]);
     if (!(p.L[d] > 0.0)) return fail(XPFC_ERR_ARG, "box length L[%d] must be positive", d);
   }
   if (p.precision != XPFC_FP64 && p.precision != XPFC_FP32_MIXED)
@@ -412,7 +422,7 @@ int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out
   c->nt_y = sbytes >= STREAM_BYTES;
   // partial buffers sized for the largest variant (narrowest tile, smallest block)
   c->npx_cap = 4096;
-  for (int tc : {4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, c->segX));
+  for (int tc : {2, 4, 8, 16}) c->npx_cap = std::max(c->npx_cap, strided_blocks(c->prec, c->n0, tc, c->segX));
   c->npz_cap = 1;
   for (int nz : {64, 128, 256, 512})
     c->npz_cap = std::max(c->npz_cap, zpass_blocks(c->prec, c->H, nz, (long long)c->n0 * c->n1));
@@ -479,7 +489,7 @@ extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
   const int prec = p->precision == XPFC_FP32_MIXED ? 1 : 0;
   const int n0 = p->n[0], n1 = p->dim == 3 ? p->n[1] : 1, n2 = p->dim == 3 ? p->n[2] : p->n[1];
   for (int v : {n0, n1, n2})
-    if (v < 1 || !is_pow2(v)) return 0;
+    if (v != 1 && !supported_side(v)) return 0;
   const int H = n2 / 2;
   if (H < 1) return 0;
   const size_t csize = prec == 1 ? 8 : 16;
@@ -489,7 +499,7 @@ extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
   sx.s[0] = Seg{0, 0, (int)xs, 1, n1 * H, 1, 0, 0};
   sx.s[1] = Seg{nyoff, 0, n1, 1, n1, 1, 0, 1};
   int npx = 4096, npz = 1;
-  for (int tc : {4, 8, 16}) npx = std::max(npx, strided_blocks(prec, n0, tc, sx));
+  for (int tc : {2, 4, 8, 16}) npx = std::max(npx, strided_blocks(prec, n0, tc, sx));
   for (int nz : {64, 128, 256, 512}) npz = std::max(npz, zpass_blocks(prec, H, nz, (long long)n0 * n1));
   // the instance's C2 table on square / cubic grids (as build_inst sizes it)
   size_t c2bytes = 0;
@@ -1267,8 +1277,8 @@ int autotune(xpfc_ctx* c) {
   // axis-0 and axis-1: tile width; real space: block (+ZFREE_REGS).  Listed in
   // order of preference (wide tiles write whole 128-byte lines); a later
   // variant must be 3 % faster to replace an earlier one.
-  const std::vector<int> xs{16, 8, 4};
-  const std::vector<int> ys{16, 16 | YRUNS_PAIRED, 8, 8 | YRUNS_PAIRED, 4, 4 | YRUNS_PAIRED};
+  const std::vector<int> xs{16, 8, 4, 2};
+  const std::vector<int> ys{16, 16 | YRUNS_PAIRED, 8, 8 | YRUNS_PAIRED, 4, 4 | YRUNS_PAIRED, 2, 2 | YRUNS_PAIRED};
   const std::vector<int> zs{256, 256 | ZFREE_REGS, 128, 128 | ZFREE_REGS, 512, 512 | ZFREE_REGS,
                             64, 64 | ZFREE_REGS};
   for (int which = 0; which < 3 && rc == XPFC_OK; ++which) {
@@ -2251,6 +2261,12 @@ extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
     if (int rc = create_impl(gp.data(), (int)gp.size(), c->stream, &q)) return bail(rc);
     q->hbud.assign((size_t)q->K, 0);
     c->cls.push_back(q);
+    // the ragged kernels compile power-of-two lines of 8 .. 1024 points (half
+    // lines up to 512)
+    if (rag_strided_wgs(q->prec, q->n0, q->segX) <= 0 || rag_zpass_wgs(q->prec, q->H, 1) <= 0 ||
+        (q->n1 > 1 && rag_strided_wgs(q->prec, q->n1, q->segY) <= 0))
+      return bail(fail(XPFC_ERR_ARG, "grid %d x %d x %d has no one-launch form (mixed-grid contexts take "
+                       "power-of-two extents of 8 .. 1024)", q->n0, q->n1, q->n2));
     c->inst_of.push_back(members[k]);
     for (size_t i = 0; i < members[k].size(); ++i) c->where[members[k][i]] = {(int)k, (int)i};
   }

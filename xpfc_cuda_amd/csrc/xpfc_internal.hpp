@@ -70,7 +70,7 @@ enum ZMode { Z_STEP = 0, Z_R2C_FLUCT = 1, Z_R2C_NL = 2, Z_C2R = 3 };
 // a*xs + b*H complex elements (input and output may use different plane
 // strides, e.g. a caller's contiguous real field and a padded spectrum).
 struct ZLines {
-  int n1;  // a power of two
+  int n1;
   long long xs_in, xs_out;
 };
 
@@ -100,7 +100,7 @@ struct Batch {
 // Register-allocation variant flag (the arithmetic is identical), or-ed into
 // the real-space workgroup size: no occupancy hint.
 constexpr int ZFREE_REGS = 1;
-// Bit 0 of an axis-1 pass variant (tile width 4, 8 or 16): workgroups dealt to
+// Bit 0 of an axis-1 pass variant (tile width 2, 4, 8 or 16): workgroups dealt to
 // the XCDs in runs of two tiles instead of one contiguous run per XCD.
 constexpr int YRUNS_PAIRED = 1;
 
@@ -149,7 +149,7 @@ struct RClass {
 constexpr int rag_tile(int L) { return L >= 8 ? RAG_NT * 8 / L : 0; }
 
 // Launch entry points of the step passes (xpfc_passes.hip).  `tc` is the tile
-// width (4, 8 or 16 columns per workgroup) of a strided pass and `ntz` the
+// width (2, 4, 8 or 16 columns per workgroup) of a strided pass and `ntz` the
 // workgroup size (64..512) of the real-space pass; a combination a size does
 // not support returns -3.  launch_xstep returns its workgroup count (= number
 // of X partials) on success.

@@ -369,17 +369,35 @@ def batch_jobs(ninst: int = 64, steps: int = 2000, seed: int = 0,
                 steps=steps, seed=i) for i in range(ninst)]
 
 
-def log_uniform_side(rng: np.random.Generator, lo: int, hi: int) -> int:
-    """Side log-uniform in [lo, hi], rounded to the nearest power of two in log2."""
+def grid_sides(lo: int, hi: int, rounding: str = "smooth") -> List[int]:
+    """The grid sides in [lo, hi] a job's side is rounded to: every extent the
+    library compiles (2^k, 3*2^k, 5*2^k) for "smooth", powers of two for "pow2"."""
+    if rounding not in ("smooth", "pow2"):
+        raise ValueError("rounding must be 'smooth' or 'pow2'")
+    out = []
+    for f in ((1, 3, 5) if rounding == "smooth" else (1,)):
+        v = 4 * f
+        while v <= hi:
+            if v >= lo and v >= 8:
+                out.append(v)
+            v *= 2
+    return sorted(out)
+
+
+def log_uniform_side(rng: np.random.Generator, lo: int, hi: int, rounding: str = "smooth") -> int:
+    """Side log-uniform in [lo, hi], rounded to the nearest grid side of
+    `grid_sides(lo, hi, rounding)` in log2."""
     e = rng.uniform(math.log2(lo), math.log2(hi))
-    return int(2 ** int(round(e)))
+    sides = grid_sides(lo, hi, rounding)
+    return int(min(sides, key=lambda v: abs(math.log2(v) - e)))
 
 
 def poisson_jobs(rate: float, duration: float, seed: int = 0, frac_3d: float = 0.5,
                  side2=(128, 1024), side3=(64, 256), steps=(500, 5000), sigma=(0.05, 0.2),
-                 n0=(0.02, 0.08)) -> List[Job]:
+                 n0=(0.02, 0.08), rounding: str = "smooth") -> List[Job]:
     """Poisson arrivals at `rate` jobs/s over `duration` s; job j draws its own
-    size, sigma, n0 and step count; its noise seed is its id."""
+    size (log-uniform side rounded as `log_uniform_side`), sigma, n0 and step
+    count; its noise seed is its id."""
     rng = np.random.default_rng(seed)
     jobs, t, jid = [], 0.0, 0
     while True:
@@ -387,7 +405,7 @@ def poisson_jobs(rate: float, duration: float, seed: int = 0, frac_3d: float = 0
         if t >= duration:
             break
         dim = 3 if rng.random() < frac_3d else 2
-        side = log_uniform_side(rng, *(side3 if dim == 3 else side2))
+        side = log_uniform_side(rng, *(side3 if dim == 3 else side2), rounding=rounding)
         jobs.append(Job(jid=jid, dim=dim, side=side, sigma=float(rng.uniform(*sigma)),
                         n0=float(rng.uniform(*n0)), steps=int(rng.integers(steps[0], steps[1] + 1)),
                         seed=jid, arrival=t))
