@@ -30,6 +30,19 @@ import time
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
+# The serving leg runs one stream per size class (17 classes with 2^k, 3*2^k
+# and 5*2^k sides).  With HIP's default of 4 hardware queues per process those
+# streams share queues and a small grid's block waits behind a large grid's
+# (measured, 30 s stream: p50 1.52 s with 4 queues, 0.64 s with 32).  The HIP
+# runtime reads the count when it starts, so it is raised (never lowered)
+# before anything touches the GPU.
+HW_QUEUES = 32
+try:
+    if int(os.environ.get("GPU_MAX_HW_QUEUES", "0")) < HW_QUEUES:
+        os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+except ValueError:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
+
 HBM_PEAK_GBS = 8000.0
 
 
@@ -98,20 +111,29 @@ def accuracy_run(n, steps=1000, seed=2):
             "reference": "FP64 GPU path on the same initial field"}
 
 
-def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
-    """The FP64 oracle (numpy/scipy pocketfft, all host threads) on a bounded
-    number of steps of the same 512^3 workload; steps/s."""
-    import numpy as np
-    from oracle import xpfc_oracle as O
-    # the host share of one GPU is 16 threads; never more than the machine has
-    workers = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
+def _oracle_fcc(O, n, sigma, t_sample=100):
     k1 = 2.0 * math.pi
     k2 = k1 * 2.0 / math.sqrt(3.0)
     L = n * 0.125
-    op = O.Params(n=(n, n, n), L=(L, L, L), sigma=0.1, psi0=0.05,
-                  peaks=[O.Peak(k1, 1.0, 1.0), O.Peak(k2, 1.0, 1.0)], dcf_mode="max",
-                  t_sample=100)
-    psi = op.psi0 + 0.01 * np.random.default_rng(2).standard_normal(op.n)
+    return O.Params(n=(n, n, n), L=(L, L, L), sigma=sigma, psi0=0.05,
+                    peaks=[O.Peak(k1, 1.0, 1.0), O.Peak(k2, 1.0, 1.0)], dcf_mode="max",
+                    t_sample=t_sample)
+
+
+def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
+    """The CPU leg: the FP64 oracle (numpy/scipy pocketfft, the host share of
+    threads) timed on a bounded number of steps of the same 512^3 workload,
+    started from the GPU's own initial field; returns the baseline record and
+    the oracle's field after `steps` steps, the checker of the GPU run."""
+    import numpy as np
+    import xpfc_cuda_amd as X
+    from oracle import xpfc_oracle as O
+    # the host share of one GPU is 16 threads; never more than the machine has
+    workers = max(1, min(int(os.environ.get("OMP_NUM_THREADS", "16")), os.cpu_count() or 1))
+    op = _oracle_fcc(O, n, 0.1)
+    with X.Context(fcc_params(n, 1)) as ctx:
+        ctx.initialize_noise(0.01, 2)
+        psi = ctx.get_field()
     st = O.Stepper(psi, op, workers)   # tables built outside the timed window
     st.step()                          # warm plans and pages
     done = 0
@@ -120,10 +142,57 @@ def cpu_baseline(n, nsteps_cap=6, budget_s=20.0):
         st.step()
         done += 1
     dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "steps/s", "cores": workers, "kind": "port",
-            "sample": "%d step(s) of the %d^3 FP64 oracle (scipy.fft, %d workers) on %s"
-                      % (done, n, workers, _cpu_model()),
-            "host_cpus": os.cpu_count()}
+    rec = {"value": done / dt, "unit": "steps/s", "cores": workers, "kind": "port",
+           "sample": "%d step(s) of the %d^3 FP64 oracle (scipy.fft, %d workers) on %s"
+                     % (done, n, workers, _cpu_model()),
+           "host_cpus": os.cpu_count()}
+    return rec, psi, st.psi, done + 1, workers
+
+
+def accuracy_vs_oracle(n, psi0, ref, nsteps, workers):
+    """The GPU against the FP64 oracle (the checker; bench.py's CPU leg only):
+    (a) the benchmark workload itself after the oracle's `nsteps` steps, from
+    the oracle's initial field psi0, FP32-mixed and FP64; (b) the same fcc physics made
+    crystallising (sigma 0.07) at 128^3, 116 steps, where the amplitude passes
+    0.1 (the nonlinear term matters): max |dn|, relative to the amplitude, and
+    the relative free-energy difference at the last sample."""
+    import numpy as np
+    import xpfc_cuda_amd as X
+    from oracle import xpfc_oracle as O
+    out = {}
+    at = {"steps": nsteps}
+    for prec, key in ((1, "fp32_mixed"), (0, "fp64")):
+        with X.Context(fcc_params(n, prec, t_sample=0)) as ctx:
+            ctx.set_field(psi0)
+            ctx.step(nsteps)
+            got = ctx.get_field()
+        at[key] = {"max_abs_dn": float(np.max(np.abs(got - ref)))}
+        del got
+    out["workload_%dcube" % n] = at
+    m, steps = 128, 116
+    op = _oracle_fcc(O, m, 0.07, t_sample=20)
+    p32 = fcc_params(m, 1, t_sample=20)
+    p32.sigma = 0.07
+    with X.Context(p32) as ctx:
+        ctx.initialize_noise(0.01, 2)
+        psi0 = ctx.get_field()
+    oref, osm = O.run(psi0, op, steps, sample=True, workers=workers)
+    amp = float(np.max(np.abs(oref - 0.05)))
+    f_or = [s.value for s in osm if s.kind == "FE" and s.step == steps][-1]
+    cr = {"steps": steps, "amplitude": amp}
+    for prec, key in ((1, "fp32_mixed"), (0, "fp64")):
+        p = fcc_params(m, prec, t_sample=20)
+        p.sigma = 0.07
+        with X.Context(p) as ctx:
+            ctx.set_field(psi0)
+            ctx.step(steps)
+            got = ctx.get_field()
+            F, _ = ctx.fe_calc()
+        dn = float(np.max(np.abs(got - oref)))
+        cr[key] = {"max_abs_dn": dn, "rel_to_amplitude": dn / amp, "fe_rel": abs(F - f_or) / abs(f_or)}
+    out["crystallising_128cube_sigma0.07"] = cr
+    out["reference"] = "FP64 oracle (numpy/scipy) on the same initial field"
+    return out
 
 
 def _cpu_model():
@@ -260,6 +329,7 @@ def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 
         rep.update({"arrival_rate": rate, "window_s": seconds, "frac_3d": frac_3d,
                     "offered_cell_updates_per_s": offered,
                     "budget_gb": budget / 1e9, "lookahead_blocks": lookahead, "lanes": lanes,
+                    "hw_queues": int(os.environ.get("GPU_MAX_HW_QUEUES", "0")),
                     "rounding": ("side log-uniform, rounded to the nearest 2^k, 3*2^k or 5*2^k in log2"
                                  if rounding == "smooth" else
                                  "side log-uniform, rounded to the nearest power of two in log2"),
@@ -406,7 +476,9 @@ def main():
                                      lookahead=args.serve_lookahead, lanes=args.serve_lanes,
                                      rounding=args.serve_rounding)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(n)
+        out["cpu_baseline"], psi0, ref, nref, workers = cpu_baseline(n)
+        out["accuracy_vs_oracle"] = accuracy_vs_oracle(n, psi0, ref, nref, workers)
+        del psi0, ref
     print(json.dumps(out), flush=True)
 
 

@@ -49,7 +49,9 @@ extern "C" {
 /* Parameters.  Field names follow read_input.cu:19-115 and globals.cuh:40-52. */
 typedef struct xpfc_params {
   int dim;                   /* 2 or 3 (the reference is 2-D, globals.cuh:26) */
-  int n[3];                  /* Mx, My, Mz (Mz = 1 in 2-D); powers of two, 8..1024 */
+  int n[3];                  /* Mx, My, Mz (Mz = 1 in 2-D): 2^k, 3*2^k or 5*2^k
+                                points (k >= 2) in 8..2048; the reference takes any
+                                Mx, My (read_input.cu:40-41) and plans cuFFT for them */
   double L[3];               /* Lx, Ly, Lz box lengths (ints in the reference) */
   double dt;                 /* time step                                       */
   double Mo;                 /* mobility                                        */
@@ -297,7 +299,8 @@ int xpfc_get_inst_params(const xpfc_ctx* ctx, int s, xpfc_params* out);
 
 /* ---- simulations of several grids in one context --------------------------- */
 /* No counterpart in the reference.  The instances may have different grids
- * (2-D and 3-D, any supported extents and box lengths) but one precision and
+ * (2-D and 3-D, power-of-two extents of 8..1024, any box lengths; other
+ * extents are refused with XPFC_ERR_ARG) but one precision and
  * device.  Every step advances all active instances of every grid with one
  * launch per pass (the axis-1 passes cover the 3-D grids), and runs of
  * unsampled steps are replayed from captured hipGraphs as in one-grid
