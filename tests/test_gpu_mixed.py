@@ -133,6 +133,13 @@ def test_mixed_rejects_operator_calls_and_mixed_precision():
         mc.initialize_noise(0, 0.01, 1)
         with pytest.raises(XPFCError):
             mc.step(1)                    # instance 1 has no field yet
+    # the ragged kernels cover power-of-two extents of 8 .. 1024: a 96^2 or a
+    # 2048^2 class is refused when the context is made, not at its first step
+    for other in ((96, 96), (2048, 64)):
+        _, c = make(other, prec=XPFC_FP64)
+        with pytest.raises(XPFCError) as e:
+            MixedContext([a, c])
+        assert "one-launch form" in str(e.value)
 
 
 @pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])

@@ -488,8 +488,7 @@ extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
     return 0;
   const int prec = p->precision == XPFC_FP32_MIXED ? 1 : 0;
   const int n0 = p->n[0], n1 = p->dim == 3 ? p->n[1] : 1, n2 = p->dim == 3 ? p->n[2] : p->n[1];
-  for (int v : {n0, n1, n2})
-    if (v != 1 && !supported_side(v)) return 0;
+  if (!supported_side(n0) || !supported_side(n2) || (p->dim == 3 && !supported_side(n1))) return 0;
   const int H = n2 / 2;
   if (H < 1) return 0;
   const size_t csize = prec == 1 ? 8 : 16;
