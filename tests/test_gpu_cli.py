@@ -71,8 +71,8 @@ LATTICE = """1          no_val
 0.05 {s}    psi_0 s
 1.0 3.0 0.0 1.0  a b eps Mo
 
-8 8        Lx Ly
-64 64      Mx My
+{lx} {ly}        Lx Ly
+{mx} {my}      Mx My
 
 30         t
 0.5        dt
@@ -98,18 +98,22 @@ def _numbers(lines):
     return out
 
 
-@pytest.mark.parametrize("iconfig,sigma,L", [(1, 0.18, 8.0), (0, 0.19, 9.0)])
-def test_cli_lines_match_the_oracle(tmp_path, iconfig, sigma, L):
+@pytest.mark.parametrize("iconfig,sigma,L,n", [(1, 0.18, (8.0, 8.0), (64, 64)), (0, 0.19, (9.0, 9.0), (64, 64)),
+                                             (1, 0.18, (12.0, 10.0), (96, 80))])
+def test_cli_lines_match_the_oracle(tmp_path, iconfig, sigma, L, n):
     """xpfc_run on a pfc.input with a lattice initial condition (square, or
     triangular with the box scaled by a_tri and truncated to int: 8 -> 9) and
     the reference's hard-coded DCF (sum of the peaks 2pi and 2sqrt2 pi, rho*B =
     4 and 4/sqrt2): every printed F and SSE_gpu value equals the FP64 oracle's
     run of the same input to the printed digits (F: 10 decimals; SSE: 6
-    significant digits)."""
+    significant digits).  The third case is a 96 x 80 grid (3*2^k and 5*2^k
+    lines) in a 12 x 10 box."""
     from oracle import xpfc_oracle as O
-    (tmp_path / "pfc.input").write_text(LATTICE.format(s=sigma, iconfig=iconfig))
+    lx = 8 if L[0] in (8.0, 9.0) else int(L[0])
+    ly = 8 if L[1] in (8.0, 9.0) else int(L[1])
+    (tmp_path / "pfc.input").write_text(LATTICE.format(s=sigma, iconfig=iconfig, lx=lx, ly=ly, mx=n[0], my=n[1]))
     lines = _run(tmp_path, "--no-vtk")
-    op = O.Params(n=(64, 64), L=(L, L), sigma=sigma, psi0=0.05, a=1.0, b=1.0 / 3.0, eps=0.0,
+    op = O.Params(n=n, L=L, sigma=sigma, psi0=0.05, a=1.0, b=1.0 / 3.0, eps=0.0,
                   M=1.0, dt=0.5, peaks=O.reference_peaks(), dcf_mode="sum", t_sample=20)
     psi = O.lattice_ic(op, iconfig, A_10=0.02, A_11=0.01, A_tri=0.02)
     _, rs = O.run(psi, op, 61, sample=True)

@@ -109,19 +109,22 @@ template <int DIR, typename C> struct Dft<16, DIR, C> {
 };
 
 // ---- radices with a factor 3 or 5 ------------------------------------------
-// cos and sin of d degrees for the angles of the radix-3/5/6/10/12/20
-// butterflies (multiples of 18 and 30 degrees), as exact decimal constants.
+// cos and sin of d degrees for the angles of the radix-3/5/6/10/12/20/24
+// butterflies (multiples of 15 and 18 degrees), as exact decimal constants.
 struct CS {
   double c, s;
 };
 __host__ __device__ constexpr CS cs_first_quadrant(int d) {
   return d == 0    ? CS{1.0, 0.0}
+         : d == 15 ? CS{0.96592582628906828674974320, 0.25881904510252076234889884}
          : d == 18 ? CS{0.95105651629515357211643933, 0.30901699437494742410229341}
          : d == 30 ? CS{0.86602540378443864676372317, 0.5}
          : d == 36 ? CS{0.80901699437494742410229341, 0.58778525229247312916870595}
+         : d == 45 ? CS{0.70710678118654752440084436, 0.70710678118654752440084436}
          : d == 54 ? CS{0.58778525229247312916870595, 0.80901699437494742410229341}
          : d == 60 ? CS{0.5, 0.86602540378443864676372317}
          : d == 72 ? CS{0.30901699437494742410229341, 0.95105651629515357211643933}
+         : d == 75 ? CS{0.25881904510252076234889884, 0.96592582628906828674974320}
          : d == 90 ? CS{0.0, 1.0}
                    : CS{0.0, 0.0};  // not an angle of these radices
 }
@@ -218,6 +221,7 @@ template <int DIR, typename C> struct Dft<6, DIR, C> : DftPQ<2, 3, DIR, C> {};
 template <int DIR, typename C> struct Dft<10, DIR, C> : DftPQ<2, 5, DIR, C> {};
 template <int DIR, typename C> struct Dft<12, DIR, C> : DftPQ<4, 3, DIR, C> {};
 template <int DIR, typename C> struct Dft<20, DIR, C> : DftPQ<4, 5, DIR, C> {};
+template <int DIR, typename C> struct Dft<24, DIR, C> : DftPQ<8, 3, DIR, C> {};
 
 // ---- stage schedule ----------------------------------------------------------
 // Each stage takes the largest radix that divides both E (a thread's points,
@@ -253,11 +257,12 @@ template <int L> constexpr int elems_per_thread() {
 }
 // Real-space half lines of H points: for 2^k, radix-16 FP32 lines from H = 128
 // up (one LDS exchange per transform at H = 128, 256) and 16 points per thread
-// at H = 1024 (the line then fills one wavefront); 12 points for 3*2^k, 20 for
-// 5*2^k.  Every choice leaves a power-of-two number of threads per line.
+// at H = 1024 (the line then fills one wavefront); for 3*2^k 12 points (24 in
+// FP32 from H = 96: 384 = 24*8*2 instead of 12*4*4*2), 20 for 5*2^k.  Every
+// choice leaves a power-of-two number of threads per line.
 template <typename T, int H> constexpr int z_elems() {
   return H % 5 == 0   ? (H < 20 ? H : 20)
-         : H % 3 == 0 ? (H < 12 ? H : 12)
+         : H % 3 == 0 ? (H < 12 ? H : (H >= 96 && sizeof(T) == 4) ? 24 : 12)
          : H < 8      ? H
          : (H >= 1024 || (H >= 128 && sizeof(T) == 4)) ? 16
                                                         : 8;

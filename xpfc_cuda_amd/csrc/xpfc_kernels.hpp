@@ -215,7 +215,9 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   else { kb = loc.col; kz = g.H; }
   const int jb = signed_index(kb, g.n1);
   const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
-  // the line's C2 values are gathered with the input streams
+  // the line's C2 values are gathered with the input streams (gathering them
+  // after the forward transform was measured slower, and for 12-point threads
+  // it raised the register count)
   double c2t[LUT ? E : 1];
   if constexpr (LUT) {
 #pragma unroll
@@ -331,26 +333,28 @@ void k_xstep(cpx<T>* __restrict__ work, cpx<T>* __restrict__ psi, Segs segs,
 template <typename T, int H> constexpr int z_tpl() { return H / z_elems<T, H>(); }  // threads per line
 // (a line's lanes must lie in one wavefront: the twiddle pairs are exchanged
 // with __shfl; every supported H = n2/2 satisfies it)
-template <typename T, int H, int NTZ> constexpr bool z_ok() {
-  return z_tpl<T, H>() <= 64 && NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0 &&
-         ((long long)(NTZ / z_tpl<T, H>()) * (H + 1) + 2 * H) * (long long)sizeof(cpx<T>) <= 152 * 1024;
-}
-// LDS line of H rows plus the X[H] slot.  For power-of-two E the rows are
-// XOR-swizzled within each block of E: a Stockham stage writes rows E*j + q
-// from lanes j, and the swizzle spreads those lanes over distinct banks; the
-// odd line length puts neighbouring lines on the other half of the banks.
-template <int H> constexpr int z_len() { return H + 1; }
+// LDS line of H rows plus the X[H] slot.  A Stockham stage writes rows E*j + q
+// from lanes j.  For power-of-two E the rows are XOR-swizzled within each
+// block of E, which spreads those lanes over distinct banks, and the odd line
+// length puts neighbouring lines on the other half of the banks; for other E
+// one padding slot follows every block of E rows (lane stride E + 1).
+template <int H, int E> constexpr int z_len() { return (E & (E - 1)) == 0 ? H + 1 : H + H / E + 1; }
+template <typename T, int H> constexpr int z_line() { return z_len<H, z_elems<T, H>()>(); }
 template <int H, int E> struct ZIdx {
   int g;
   __device__ __forceinline__ int operator()(int r) const {
     if constexpr ((E & (E - 1)) == 0) {
       const int s = (r >= H) ? r : (r ^ ((r / E) & (E - 1)));
-      return g * z_len<H>() + s;
+      return g * z_len<H, E>() + s;
     } else {
-      return g * z_len<H>() + r;
+      return g * z_len<H, E>() + r + r / E;
     }
   }
 };
+template <typename T, int H, int NTZ> constexpr bool z_ok() {
+  return z_tpl<T, H>() <= 64 && NTZ >= z_tpl<T, H>() && NTZ % z_tpl<T, H>() == 0 &&
+         ((long long)(NTZ / z_tpl<T, H>()) * z_line<T, H>() + 2 * H) * (long long)sizeof(cpx<T>) <= 152 * 1024;
+}
 
 // Free-energy sums (psi, psi^2, psi^3, psi^4) of two real-space values, and
 // the nonlinear term N(psi) = -a psi^2 + b psi^3.  F32: the powers are those
@@ -403,9 +407,9 @@ __device__ __forceinline__ void with_powers(const ZArgs& za, F&& body) {
 // power of two <= 64, so a line never straddles two wavefronts).
 // FREE: no occupancy hint (the compiler may use more registers instead of
 // spilling); a register-allocation variant only, the arithmetic is identical.
-// LDS of the real-space pass: G lines of z_len<H> plus the two H-entry tables.
+// LDS of the real-space pass: G lines of z_line<T, H> plus the two H-entry tables.
 template <typename T, int H, int NTZ> constexpr int z_lds_elems() {
-  return (NTZ / z_tpl<T, H>()) * z_len<H>() + 2 * H;
+  return (NTZ / z_tpl<T, H>()) * z_line<T, H>() + 2 * H;
 }
 
 // The lines of workgroup `wb` of one instance (pointers already at the
@@ -421,7 +425,7 @@ __device__ __forceinline__ void zpass_body(const void* __restrict__ in, const cp
   constexpr int TPL = z_tpl<T, H>();
   static_assert(TPL <= 64 && (TPL & (TPL - 1)) == 0, "a line must lie in one wavefront");
   constexpr int G = NTZ / TPL;
-  C* twl = lds + G * z_len<H>();  // the H-point FFT twiddles
+  C* twl = lds + G * z_line<T, H>();  // the H-point FFT twiddles
   C* twpl = twl + H;              // and the half-spectrum twiddles
   stage_table<NTZ>(twl, tw, H);
   stage_table<NTZ>(twpl, twp, H);
