@@ -62,6 +62,11 @@ def spectrum_bytes(n, prec):
     return n * n * (n // 2 + 1) * csize
 
 
+def copy_ceiling(nbytes):
+    import xpfc_cuda_amd as X
+    return max(X.copy_bandwidth(nbytes, reps=20) for _ in range(3))
+
+
 def gpu_run(n, prec, steps, warmup, seed=2, pin=None):
     import xpfc_cuda_amd as X
     p = fcc_params(n, prec)
@@ -441,6 +446,11 @@ def main():
                      "algorithmic_bytes": 4 * S},
         "samples": [s.line().strip() for s in samples[:4]],
     }
+    # the practical ceiling: a plain streaming copy of one spectrum (read + write)
+    copy = copy_ceiling(S)
+    out["roofline"]["copy_gbs"] = round(copy, 1)
+    out["roofline"]["frac_of_copy"] = round(achieved / copy, 4)
+    out["step_frac_of_copy"] = round(step_gbs / copy, 4)
     if not args.no_fp64:
         fsteps = max(10, args.steps // 4)
         ms64, _, passes64, _, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))

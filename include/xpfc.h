@@ -161,7 +161,13 @@ int xpfc_time_steps(xpfc_ctx* ctx, int nsteps, float* ms);
  * pass, the real-space pass and the axis-1 forward pass (0 in 2-D), ms[4] =
  * average ms per whole step. */
 int xpfc_time_passes(xpfc_ctx* ctx, int nsteps, double* ms);
-/* Pass variants: v[0] = tile width (columns per workgroup, 4/8/16) of the
+/* The HBM rate of a plain streaming copy on `device`: a kernel that only
+ * copies `bytes` (16 bytes per lane per access) between two fresh buffers,
+ * timed with hipEvents over `reps` copies after one warm-up; *gbs = read +
+ * written bytes per second / 1e9.  The practical ceiling the passes are
+ * compared with (beside the 8 TB/s peak).  No counterpart in the reference. */
+int xpfc_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
+/* Pass variants: v[0] = tile width (columns per workgroup, 2/4/8/16) of the
  * axis-0 pass, v[1] = that of the axis-1 passes (plus 1: workgroups dealt to
  * the XCDs in runs of two tiles), v[2] = workgroup size (64..512) of the
  * real-space pass (plus 1: no occupancy hint).  Variants regroup lines into

@@ -363,3 +363,13 @@ def test_fcc_64cube_fp32_1000_steps(sigma):
     for s in rs:
         if s.kind == "FE":
             assert abs(gfe[s.step] - s.value) <= 1e-7 * abs(s.value), (s.step, gfe[s.step], s.value)
+
+
+def test_copy_bandwidth_is_a_plausible_hbm_rate():
+    """The streaming-copy reference the bench compares the passes with: a
+    positive rate below the 8 TB/s peak, stable across calls."""
+    import xpfc_cuda_amd as X
+    rates = [X.copy_bandwidth(1 << 28, reps=10) for _ in range(3)]
+    for r in rates:
+        assert 1000.0 < r < 8000.0, rates
+    assert max(rates) < 1.2 * min(rates), rates

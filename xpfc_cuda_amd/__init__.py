@@ -9,10 +9,10 @@ through that library; if it is missing, importing the bindings raises.
 from .api import (  # noqa: F401
     XPFC_FP64, XPFC_FP32_MIXED, XPFC_DCF_SUM, XPFC_DCF_MAX,
     XPFCError, Params, Sample, Context, BatchContext, MixedContext, lib, load_library, library_path,
-    read_input, randn, philox4x32_10, reference_steps, instance_bytes,
+    read_input, randn, philox4x32_10, reference_steps, instance_bytes, copy_bandwidth,
 )
 
 __all__ = ["XPFC_FP64", "XPFC_FP32_MIXED", "XPFC_DCF_SUM", "XPFC_DCF_MAX",
            "XPFCError", "Params", "Sample", "Context", "BatchContext", "MixedContext", "lib", "load_library",
            "library_path", "read_input", "randn", "philox4x32_10", "reference_steps",
-           "instance_bytes"]
+           "instance_bytes", "copy_bandwidth"]

@@ -128,6 +128,7 @@ def load_library(path: Optional[str] = None):
         "xpfc_time_steps": (i, [vp, i, P(ct.c_float)]),
         "xpfc_sync": (i, [vp]),
         "xpfc_time_passes": (i, [vp, i, P(d)]),
+        "xpfc_copy_bandwidth": (i, [i, ct.c_size_t, i, P(d)]),
         "xpfc_autotune": (i, [vp, P(i)]),
         "xpfc_set_variants": (i, [vp, P(i)]),
         "xpfc_get_variants": (i, [vp, P(i)]),
@@ -340,6 +341,14 @@ def instance_bytes(p: Params) -> int:
     """Device bytes one instance of this grid and precision occupies."""
     c = p.to_c()
     return int(lib().xpfc_instance_bytes(ct.byref(c)))
+
+
+def copy_bandwidth(nbytes: int, reps: int = 20, device: int = 0) -> float:
+    """GB/s (read + written) of a plain streaming copy of `nbytes` on the GPU:
+    the practical HBM ceiling the passes are compared with."""
+    g = ct.c_double()
+    _check(lib().xpfc_copy_bandwidth(int(device), int(nbytes), int(reps), ct.byref(g)), "xpfc_copy_bandwidth")
+    return g.value
 
 
 def randn(seed: int, count: int, first: int = 0) -> np.ndarray:

@@ -1451,6 +1451,40 @@ extern "C" int xpfc_time_steps(xpfc_ctx* c, int nsteps, float* ms) {
   return c->cls.empty() ? drain_report(c) : mixed_sync(c);
 }
 
+extern "C" int xpfc_copy_bandwidth(int device, size_t bytes, int reps, double* gbs) {
+  if (!gbs || reps < 1 || bytes < 16) return fail(XPFC_ERR_ARG, "bad arguments");
+  bytes &= ~(size_t)15;
+  if (hipSetDevice(device) != hipSuccess) return fail(XPFC_ERR_DEVICE, "hipSetDevice(%d) failed", device);
+  struct Res {
+    void *a = nullptr, *b = nullptr;
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Res() {
+      if (s) (void)hipStreamSynchronize(s);
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+      if (a) (void)hipFree(a);
+      if (b) (void)hipFree(b);
+      if (s) (void)hipStreamDestroy(s);
+    }
+  } r;
+  if (hipMalloc(&r.a, bytes) != hipSuccess || hipMalloc(&r.b, bytes) != hipSuccess)
+    return fail(XPFC_ERR_NOMEM, "device allocation of 2 x %zu bytes failed", bytes);
+  HIPCHK(hipStreamCreateWithFlags(&r.s, hipStreamNonBlocking));
+  HIPCHK(hipEventCreate(&r.e0));
+  HIPCHK(hipEventCreate(&r.e1));
+  HIPCHK(hipMemsetAsync(r.a, 0, bytes, r.s));
+  LCHK(launch_stream_copy(r.a, r.b, bytes, r.s));  // warm-up
+  HIPCHK(hipEventRecord(r.e0, r.s));
+  for (int i = 0; i < reps; ++i) LCHK(launch_stream_copy(r.a, r.b, bytes, r.s));
+  HIPCHK(hipEventRecord(r.e1, r.s));
+  HIPCHK(hipEventSynchronize(r.e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, r.e0, r.e1));
+  *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+  return XPFC_OK;
+}
+
 extern "C" int xpfc_time_passes(xpfc_ctx* c, int nsteps, double* ms) {
   XPFC_NOT_MIXED(c, "xpfc_time_passes");
   if (!c || !ms || nsteps <= 0) return fail(XPFC_ERR_ARG, "bad arguments");

@@ -224,6 +224,8 @@ int launch_noise_sum(long long n, unsigned long long seed, double* partial, int*
 int launch_noise_fill(int prec, void* out, long long n, long long plane, long long pitch,
                       unsigned long long seed, double amp, double mean_z, hipStream_t s);
 // flag[s] = 1 if instance s's spectrum (at s * stride) is finite at two probe modes
+// plain streaming copy of `bytes` (multiple of 16) from src to dst
+int launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 int launch_probe(int prec, const void* psi, long long stride, long long iny, int K, int* flag,
                  hipStream_t s);
 int launch_from_double(int prec, const double* src, void* dst, long long n, hipStream_t s);

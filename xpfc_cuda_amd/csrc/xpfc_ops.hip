@@ -384,4 +384,21 @@ int launch_from_double(int prec, const double* src, void* dst, long long n, hipS
   return launch_status();
 }
 
+// ---------------------------------------------------------------------------
+// plain streaming copy: the HBM rate of a kernel that only moves data
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_stream_copy(const float4* __restrict__ src, float4* __restrict__ dst,
+                                                     long long n) {
+  for (long long i = (long long)blockIdx.x * 256 + threadIdx.x; i < n; i += (long long)gridDim.x * 256)
+    dst[i] = src[i];
+}
+
+int launch_stream_copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+  const long long n = (long long)(bytes / sizeof(float4));
+  if (n <= 0) return 0;
+  const long long nb = std::min<long long>((n + 255) / 256, 32768);
+  hipLaunchKernelGGL(k_stream_copy, dim3((unsigned)nb), dim3(256), 0, s, (const float4*)src, (float4*)dst, n);
+  return launch_status();
+}
+
 }  // namespace xpfc
