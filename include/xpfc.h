@@ -305,10 +305,11 @@ int xpfc_get_inst_params(const xpfc_ctx* ctx, int s, xpfc_params* out);
 
 /* ---- simulations of several grids in one context --------------------------- */
 /* No counterpart in the reference.  The instances may have different grids
- * (2-D and 3-D, power-of-two extents of 8..1024, any box lengths; other
- * extents are refused with XPFC_ERR_ARG) but one precision and
- * device.  Every step advances all active instances of every grid with one
- * launch per pass (the axis-1 passes cover the 3-D grids), and runs of
+ * (2-D and 3-D, extents of 2^k, 3*2^k or 5*2^k points up to 1024, any box
+ * lengths; longer extents are refused with XPFC_ERR_ARG) but one precision
+ * and device.  Every step advances all active instances of every grid with
+ * one launch per pass and line-length family present (the axis-1 passes cover
+ * the 3-D grids), and runs of
  * unsampled steps are replayed from captured hipGraphs as in one-grid
  * contexts, so a block of steps of the whole mix is one graph.  Each
  * instance's field is bitwise the one a one-instance context gives; sampled

@@ -84,6 +84,53 @@ def test_mixed_equals_alone(prec, graphs):
     assert not np.array_equal(got[0][0], got[2][0])  # same grid, own physics
 
 
+def _families(prec):
+    """Grids whose passes fall in every line family (2^k, 3*2^k, 5*2^k
+    points), at each family's shortest and longest ragged lines, with square /
+    cubic boxes and others, mixed with power-of-two grids in one context."""
+    ps = []
+    for shape, kw in (((96, 96), dict(sigma=0.08, t_sample=7)),            # 96-point lines, 48-point half lines
+                      ((80, 160), dict(sigma=0.1, t_sample=5, dt=0.3)),      # 5*2^k, non-square box
+                      ((48, 40, 24), dict(sigma=0.09, t_sample=4)),         # axis 0 3*2^k, axis 1 5*2^k, H = 12
+                      ((20, 24, 40), dict(sigma=0.12, t_sample=0)),         # axis 0 5*2^k, axis 1 3*2^k, H = 20
+                      ((768, 12), dict(sigma=0.07, t_sample=10)),           # longest 3*2^k line, H = 6
+                      ((640, 20), dict(sigma=0.11, t_sample=6)),            # longest 5*2^k line, H = 10
+                      ((12, 768), dict(sigma=0.08, t_sample=9)),            # shortest 3*2^k line, H = 384
+                      ((20, 640), dict(sigma=0.1, t_sample=3)),             # shortest 5*2^k line, H = 320
+                      ((96, 96, 96), dict(sigma=0.09, t_sample=8)),         # cubic, every axis 3*2^k
+                      ((64, 64), dict(sigma=0.07, t_sample=7)),
+                      ((16, 16, 32), dict(sigma=0.1, t_sample=5))):
+        _, p = make(shape, prec=prec, psi0=0.05, **kw)
+        ps.append(p)
+    return ps
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_mixed_line_families_equal_alone(prec, graphs):
+    _flush("mixed line families vs alone prec=%d graphs=%d" % (prec, graphs))
+    ps = _families(prec)
+    psis = [_field(tuple(p.n), 70 + s) for s, p in enumerate(ps)]
+    nsteps = 23
+    with MixedContext(ps) as mc:
+        mc.set_graphs(graphs)
+        for s, x in enumerate(psis):
+            mc.set_field(s, x)
+        mc.step(nsteps)
+        mc.set_active([2, 5, 8])          # three families, some grids idle
+        mc.step(4)
+        mc.set_active(None)
+        mc.step(3)
+        mc.sync()
+        got = [(mc.get_field(s), mc.current_step(s)) for s in range(len(ps))]
+    for s, p in enumerate(ps):
+        n = nsteps + 3 + (4 if s in (2, 5, 8) else 0)
+        f, _, _ = _alone(p, psis[s], n)
+        assert np.all(np.isfinite(f))
+        assert got[s][1] == n
+        assert np.array_equal(got[s][0], f), "instance %d (%s) differs" % (s, p.n)
+
+
 @pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
 def test_mixed_active_subsets_and_slot_reuse(prec):
     _flush("mixed active subsets prec=%d" % prec)
@@ -133,9 +180,10 @@ def test_mixed_rejects_operator_calls_and_mixed_precision():
         mc.initialize_noise(0, 0.01, 1)
         with pytest.raises(XPFCError):
             mc.step(1)                    # instance 1 has no field yet
-    # the ragged kernels cover power-of-two extents of 8 .. 1024: a 96^2 or a
-    # 2048^2 class is refused when the context is made, not at its first step
-    for other in ((96, 96), (2048, 64)):
+    # the ragged kernels cover extents of 2^k, 3*2^k and 5*2^k points up to
+    # 1024 (half lines up to 512): a longer line is refused when the context
+    # is made, not at its first step
+    for other in ((2048, 64), (1536, 48), (1280, 20), (64, 2048), (16, 8, 1280)):
         _, c = make(other, prec=XPFC_FP64)
         with pytest.raises(XPFCError) as e:
             MixedContext([a, c])

@@ -30,7 +30,8 @@ HIPFLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contra
 # the step passes of each line length compile in their own units (xpfc_len_*.hip), in parallel
 LEN_UNITS = ["xpfc_len_p2a.hip", "xpfc_len_p2b.hip", "xpfc_len_p2c.hip", "xpfc_len_p2d.hip",
              "xpfc_len_r3a.hip", "xpfc_len_r3b.hip", "xpfc_len_r5a.hip", "xpfc_len_r5b.hip"]
-SOURCES = LEN_UNITS + ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp",
+RAG_UNITS = ["xpfc_rag_p2.hip", "xpfc_rag_r3.hip", "xpfc_rag_r5.hip"]
+SOURCES = LEN_UNITS + RAG_UNITS + ["xpfc_passes.hip", "xpfc_ops.hip", "xpfc_params.cpp", "xpfc_context.cpp",
                        "xpfc_sched.cpp", "xpfc_service.cpp"]
 
 

@@ -1928,6 +1928,31 @@ int rag_lgc(int n) {
 }
 int round8(int n) { return (n + 7) & ~7; }
 
+// Groups the classes of one pass into launches by (line family, LUT form),
+// numbering each launch's workgroups from 0 (wg0 of the strided passes on a
+// multiple of 8, so a workgroup's XCD is its index in the class mod 8).
+void rag_group(std::vector<std::pair<int, RClass>>& in, bool strided, std::vector<RClass>& all,
+               std::vector<xpfc_ctx::RagLaunch>& out, int prec) {
+  for (int fam : {1, 3, 5}) {
+    for (int lut : {1, 0}) {
+      xpfc_ctx::RagLaunch l;
+      l.fam = fam;
+      l.lut = lut != 0;
+      l.first = (int)all.size();
+      for (auto& e : in) {
+        RClass r = e.second;
+        if (line_family(r.L) != fam || (r.cubic != 0) != l.lut) continue;
+        r.wg0 = strided ? round8(l.nwg) : l.nwg;
+        l.nwg = r.wg0 + r.nact * r.wgs;
+        l.lds = std::max(l.lds, strided ? rag_strided_lds(prec, r.L) : rag_zpass_lds(prec, r.L));
+        all.push_back(r);
+        ++l.ncls;
+      }
+      if (l.ncls) out.push_back(l);
+    }
+  }
+}
+
 // The class tables of the current active lists (built once per distinct
 // list lengths; the kernels read the lists themselves from device memory).
 int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
@@ -1939,7 +1964,7 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     return XPFC_OK;
   }
   xpfc_ctx::RagTables t;
-  std::vector<RClass> X, XB, Y, Z;  // axis 0 with C2 tables (square / cubic boxes), axis 0 otherwise
+  std::vector<std::pair<int, RClass>> X, Y, Z;  // (class, entry) of each pass
   const int nc = (int)c->cls.size();
   t.nbx.assign(nc, 0);
   t.nbz.assign(nc, 0);
@@ -1949,14 +1974,12 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     t.nbx[k] = rag_strided_wgs(q->prec, q->n0, q->segX);
     t.nbz[k] = rag_zpass_wgs(q->prec, q->H, (long long)q->n0 * q->n1);
     const int nby = q->n1 > 1 ? rag_strided_wgs(q->prec, q->n1, q->segY) : 0;
-    if (t.nbx[k] <= 0 || t.nbz[k] <= 0 || nby < 0 || q->H > 512)
+    if (t.nbx[k] <= 0 || t.nbz[k] <= 0 || nby < 0)
       return fail(XPFC_ERR_ARG, "grid %d x %d x %d has no one-launch form", q->n0, q->n1, q->n2);
     if (na == 0) continue;
     const Batch bt = bt_active(q);
-    int& wx = q->g.cubic ? t.wx : t.wxb;
     RClass r{};
     r.L = q->n0;
-    r.wg0 = wx = round8(wx);
     r.lgc = rag_lgc(na * t.nbx[k]);
     r.wgs = t.nbx[k];
     r.nact = na;
@@ -1969,13 +1992,10 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     r.tw = q->tw0;
     r.partial = q->px;
     r.budget = budget ? q->d_budget : nullptr;
-    (q->g.cubic ? X : XB).push_back(r);
-    wx += na * r.wgs;
-    t.lx = std::max(t.lx, rag_strided_lds(q->prec, q->n0));
+    X.push_back({k, r});
     if (q->n1 > 1) {
       RClass y{};
       y.L = q->n1;
-      y.wg0 = t.wy = round8(t.wy);
       y.lgc = -1;
       y.wgs = nby;
       y.nact = na;
@@ -1984,13 +2004,10 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
       y.a = q->work;
       y.tw = q->tw1;
       y.budget = r.budget;
-      Y.push_back(y);
-      t.wy += na * nby;
-      t.ly = std::max(t.ly, rag_strided_lds(q->prec, q->n1));
+      Y.push_back({k, y});
     }
     RClass z{};
     z.L = q->H;
-    z.wg0 = t.wz;
     z.wgs = t.nbz[k];
     z.nact = na;
     z.pw = q->any_powf;
@@ -2003,20 +2020,14 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     z.zl = ZLines{q->n1, q->xs, q->xs};
     z.nyoff = q->nyoff;
     z.budget = r.budget;
-    Z.push_back(z);
-    t.wz += na * z.wgs;
-    t.lz = std::max(t.lz, rag_zpass_lds(q->prec, q->H));
+    Z.push_back({k, z});
   }
-  t.nx = (int)X.size();
-  t.nxb = (int)XB.size();
-  t.ny = (int)Y.size();
-  t.nz = (int)Z.size();
-  if (t.nz > RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids active at once", RAG_MAX);
-  LCHK(rag_prepare(c->prec, t.lx, t.ly, t.lz));
-  std::vector<RClass> all(X);
-  all.insert(all.end(), XB.begin(), XB.end());
-  all.insert(all.end(), Y.begin(), Y.end());
-  all.insert(all.end(), Z.begin(), Z.end());
+  if ((int)Z.size() > RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids active at once", RAG_MAX);
+  std::vector<RClass> all;
+  rag_group(X, true, all, t.x, c->prec);
+  rag_group(Y, true, all, t.y, c->prec);
+  rag_group(Z, false, all, t.z, c->prec);
+  for (int fam : {1, 3, 5}) LCHK(rag_prepare(c->prec, fam));
   if (!all.empty()) {
     if (hipMalloc((void**)&t.d, all.size() * sizeof(RClass)) != hipSuccess)
       return fail(XPFC_ERR_NOMEM, "class table allocation failed");
@@ -2035,21 +2046,21 @@ bool mixed_pw(const xpfc_ctx* c) {
   return false;
 }
 
-// The launches of one step of every active instance of every class: axis 0
-// (a second launch only when square / cubic and other boxes are mixed), axis
-// 1 inverse (3-D classes), real space, axis 1 forward (3-D classes),
-// then for sampled steps each listed class's finalize into its row `rows[k]`.
+// The launches of one step of every active instance of every class, one per
+// line family present in the pass: axis 0 (separate launches for square /
+// cubic and other boxes), axis 1 inverse (3-D classes), real space, axis 1
+// forward (3-D classes), then for sampled steps each listed class's finalize
+// into its row `rows[k]`.
 int mixed_one_step(xpfc_ctx* c, const xpfc_ctx::RagTables& t, bool sample, int sib,
                    const std::vector<int>* rows, bool pw) {
-  const RClass* X = t.d;
-  const RClass* XB = X + t.nx;
-  const RClass* Y = XB + t.nxb;
-  const RClass* Z = Y + t.ny;
-  if (t.nx) LCHK(launch_xstep_ragged(c->prec, X, t.nx, t.wx, t.lx, true, sample, sib, c->stream));
-  if (t.nxb) LCHK(launch_xstep_ragged(c->prec, XB, t.nxb, t.wxb, t.lx, false, sample, sib, c->stream));
-  if (t.ny) LCHK(launch_strided_ragged(c->prec, Y, t.ny, t.wy, t.ly, +1, sib, c->stream));
-  LCHK(launch_zpass_ragged(c->prec, Z, t.nz, t.wz, t.lz, sample, pw, sib, c->stream));
-  if (t.ny) LCHK(launch_strided_ragged(c->prec, Y, t.ny, t.wy, t.ly, -1, sib, c->stream));
+  for (const auto& l : t.x)
+    LCHK(launch_xstep_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, l.lut, sample, sib, c->stream));
+  for (const auto& l : t.y)
+    LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, +1, sib, c->stream));
+  for (const auto& l : t.z)
+    LCHK(launch_zpass_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, sample, pw, sib, c->stream));
+  for (const auto& l : t.y)
+    LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, -1, sib, c->stream));
   if (sample && rows) {
     for (size_t k = 0; k < c->cls.size(); ++k) {
       if ((*rows)[k] < 0) continue;
@@ -2294,12 +2305,12 @@ extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
     if (int rc = create_impl(gp.data(), (int)gp.size(), c->stream, &q)) return bail(rc);
     q->hbud.assign((size_t)q->K, 0);
     c->cls.push_back(q);
-    // the ragged kernels compile power-of-two lines of 8 .. 1024 points (half
-    // lines up to 512)
+    // the ragged kernels compile lines of 8 .. 1024 points (half lines up to
+    // 512) of each family
     if (rag_strided_wgs(q->prec, q->n0, q->segX) <= 0 || rag_zpass_wgs(q->prec, q->H, 1) <= 0 ||
         (q->n1 > 1 && rag_strided_wgs(q->prec, q->n1, q->segY) <= 0))
       return bail(fail(XPFC_ERR_ARG, "grid %d x %d x %d has no one-launch form (mixed-grid contexts take "
-                       "power-of-two extents of 8 .. 1024)", q->n0, q->n1, q->n2));
+                       "extents of 2^k, 3*2^k or 5*2^k points up to 1024)", q->n0, q->n1, q->n2));
     c->inst_of.push_back(members[k]);
     for (size_t i = 0; i < members[k].size(); ++i) c->where[members[k][i]] = {(int)k, (int)i};
   }

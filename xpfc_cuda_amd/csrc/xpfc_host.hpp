@@ -115,11 +115,16 @@ struct xpfc_ctx {
   std::vector<std::pair<int, int>> where;
   std::vector<std::vector<int>> inst_of;  // class k, slot i -> instance number
   bool borrowed_stream = false;      // a class context: the stream belongs to its owner
+  struct RagLaunch {                 // one ragged launch: classes of one line family
+    int fam = 1;                     // 1, 3 or 5 (2^k, 3*2^k, 5*2^k points)
+    bool lut = false;                // axis 0: every class has a square / cubic box
+    int first = 0, ncls = 0;         // its classes: d[first .. first + ncls)
+    int nwg = 0;                     // workgroups
+    size_t lds = 0;                  // dynamic LDS bytes
+  };
   struct RagTables {                 // device class tables of one active-count configuration
-    xpfc::RClass* d = nullptr;       // x (square / cubic boxes) | x (others) | y | z, back to back
-    int nx = 0, nxb = 0, ny = 0, nz = 0;  // classes in each
-    int wx = 0, wxb = 0, wy = 0, wz = 0;  // workgroups of each launch
-    size_t lx = 0, ly = 0, lz = 0;   // dynamic LDS bytes of each kind of launch
+    xpfc::RClass* d = nullptr;       // the classes of every launch, back to back
+    std::vector<RagLaunch> x, y, z;  // axis 0, axis 1 (3-D classes), real space
     std::vector<int> nbx, nbz;       // per class: X / Z partial sums per instance
   };
   std::map<std::vector<int>, RagTables> rtab;                  // key: active counts (+ budget flag)

@@ -114,13 +114,15 @@ constexpr int NSUMS = 6;
 // Ragged launches: one launch per pass over the instances of several grids
 // (size classes).  Workgroups [wg0, wg0 + nact * wgs) of the launch belong to
 // class k, instance-major; a workgroup finds its class by scanning wg0.  The
-// workgroup size is fixed per pass (RAG_NT for the strided passes, RAG_NTZ
-// for the real-space pass), so a class with lines of L points takes tiles of
-// RAG_NT * 8 / L columns (RAG_NTZ / (threads per line) lines); the arithmetic
-// of every line is that of the one-grid kernels.
+// kernels are compiled per line-length family (2^k, 3*2^k, 5*2^k points), so
+// a pass over classes of several families is one launch per family.  The
+// workgroup size is fixed per family and pass (rag_nt for the strided passes,
+// RAG_NTZ for the real-space pass), so a class with lines of L points takes
+// tiles of rag_nt * E / L columns (E points per thread) and RAG_NTZ /
+// (threads per line) lines; the arithmetic of every line is that of the
+// one-grid kernels.
 // ---------------------------------------------------------------------------
 constexpr int RAG_MAX = 16;    // classes per ragged launch
-constexpr int RAG_NT = 512;    // threads per workgroup, strided passes
 constexpr int RAG_NTZ = 256;   // threads per workgroup, real-space pass
 struct RClass {
   int L;            // line length of the pass (n0, n1, or H = n2 / 2)
@@ -145,8 +147,34 @@ struct RClass {
   long long nyoff;  // real space: offset of the Nyquist plane (complex elements)
   const int* budget;  // steps of the current block per slot (nullptr: every step)
 };
+// Family of a line length: 1 (2^k), 3 (3*2^k) or 5 (5*2^k points).
+constexpr int line_family(int L) { return L % 5 == 0 ? 5 : L % 3 == 0 ? 3 : 1; }
+// Points per thread of a strided line (the FFT's per-thread radix budget).
+constexpr int line_elems(int L) {
+  return L % 5 == 0 ? (L < 20 ? L : 20) : L % 3 == 0 ? (L < 12 ? L : 12) : (L < 8 ? L : 8);
+}
+// Threads per workgroup of a family's ragged strided passes.  12- and
+// 20-point lines take 256 (one wave per SIMD, so the axis-0 pass may hold up
+// to 512 registers per lane: at 512 threads its FP64 form spills; 20-point FP64
+// tiles of 512 threads would also not fit the LDS).
+constexpr int rag_nt(int fam) { return fam == 1 ? 512 : 256; }
 // Column tile width of a ragged strided pass over lines of L points.
-constexpr int rag_tile(int L) { return L >= 8 ? RAG_NT * 8 / L : 0; }
+constexpr int rag_tile(int L) {
+  return L >= 8 ? rag_nt(line_family(L)) * line_elems(L) / L : 0;
+}
+// The ragged launchers of one precision and family (xpfc_ragged.hpp,
+// instantiated by xpfc_rag_*.hip).
+template <typename T, int FAM>
+struct RagOps {
+  static int prepare();
+  static int xstep(const RClass* cls, int ncls, int nwg, size_t lds, bool lut, bool sample, int sib,
+                   hipStream_t s);
+  static int strided(const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib, hipStream_t s);
+  static int zpass(const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw, int sib,
+                   hipStream_t s);
+  static int strided_wgs(int L, const Segs& segs);
+  static long long zpass_wgs(int H, long long nlines, size_t* lds);
+};
 
 // Launch entry points of the step passes (xpfc_passes.hip).  `tc` is the tile
 // width (2, 4, 8 or 16 columns per workgroup) of a strided pass and `ntz` the
@@ -171,21 +199,22 @@ int launch_zpass(int prec, int mode, const void* in, const void* in_ny, void* ou
                  const void* tw, const void* twp, double* partial, bool sample, bool pw,
                  hipStream_t s);
 int zpass_blocks(int prec, int H, int ntz, long long nlines);
-// Ragged launches over the device table `cls` (ncls classes, `nwg` workgroups
-// in all, `lds` bytes of dynamic LDS).  `sib`: index of the step within its
-// block, compared with the per-slot budgets.  Return 0, or -2 / -3 as above.
-// (`lut`: every class of this table has a square / cubic box)
-int launch_xstep_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool lut, bool sample,
-                        int sib, hipStream_t s);
-int launch_strided_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib,
-                          hipStream_t s);
-int launch_zpass_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw,
-                        int sib, hipStream_t s);
+// Ragged launches over the device table `cls` (ncls classes of line family
+// `fam`, `nwg` workgroups in all, `lds` bytes of dynamic LDS).  `sib`: index
+// of the step within its block, compared with the per-slot budgets.  Return
+// 0, or -2 / -3 as above.  (`lut`: every class of this table has a square /
+// cubic box)
+int launch_xstep_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, bool lut,
+                        bool sample, int sib, hipStream_t s);
+int launch_strided_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, int dir,
+                          int sib, hipStream_t s);
+int launch_zpass_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, bool sample,
+                        bool pw, int sib, hipStream_t s);
+// Lets the ragged kernels of precision prec and family fam use the dynamic
+// LDS of the family's longest tiles (call before launching or capturing them).
+int rag_prepare(int prec, int fam);
 // Workgroups per instance and dynamic LDS bytes of a class in a ragged launch
-// (0 / negative: lines of that length are not supported).
-// Lets the ragged kernels of precision prec use the given dynamic LDS (call
-// before launching or capturing them).
-int rag_prepare(int prec, size_t lx, size_t ly, size_t lz);
+// (0 / negative: lines of that length have no ragged form).
 int rag_strided_wgs(int prec, int L, const Segs& segs);
 size_t rag_strided_lds(int prec, int L);
 int rag_zpass_wgs(int prec, int H, long long nlines);

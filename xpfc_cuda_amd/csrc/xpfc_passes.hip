@@ -1,8 +1,10 @@
 // HIP kernels of the XPFC time step for gfx950 (MI355X): the dispatch of the
 // step passes on the runtime line length (the per-length launchers are
 // compiled in xpfc_len_*.hip), the fixed-order reductions, the C2 table, and
-// the ragged launches that step instances of several grids in one launch per
-// pass.  The pass kernels themselves are in xpfc_kernels.hpp.
+// the dispatch of the ragged launches that step instances of several grids in
+// one launch per pass and line family (compiled in xpfc_rag_*.hip).  The pass
+// kernels themselves are in xpfc_kernels.hpp and xpfc_ragged.hpp.
+#include <climits>
 #include <type_traits>
 
 #include "xpfc_kernels.hpp"
@@ -221,229 +223,85 @@ int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream
 }
 
 // ---------------------------------------------------------------------------
-// ragged launches: several size classes in one launch per pass
+// ragged launches: several size classes in one launch per pass and line
+// family (kernels in xpfc_ragged.hpp, one unit per family)
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int rag_class(const RClass* __restrict__ cls, int ncls) {
-  int k = 0;
-  for (int i = 1; i < ncls; ++i)
-    if ((int)blockIdx.x >= cls[i].wg0) k = i;
-  return k;
-}
-
-#define XPFC_RLSWITCH(L, ...)                                    \
-  switch (L) {                                                   \
-    case 8: { constexpr int LL = 8; __VA_ARGS__; } break;        \
-    case 16: { constexpr int LL = 16; __VA_ARGS__; } break;      \
-    case 32: { constexpr int LL = 32; __VA_ARGS__; } break;      \
-    case 64: { constexpr int LL = 64; __VA_ARGS__; } break;      \
-    case 128: { constexpr int LL = 128; __VA_ARGS__; } break;    \
-    case 256: { constexpr int LL = 256; __VA_ARGS__; } break;    \
-    case 512: { constexpr int LL = 512; __VA_ARGS__; } break;    \
-    case 1024: { constexpr int LL = 1024; __VA_ARGS__; } break;  \
-    default: break;                                              \
-  }
-
-// LUT: every class of the launch has a square / cubic box (C2 from the
-// table); the other form evaluates C2 per mode and needs more registers, so
-// the two never share a kernel.
-template <typename T, bool SAMPLE, bool LUT>
-__global__ __launch_bounds__(RAG_NT) void k_xstep_r(const RClass* __restrict__ cls, int ncls, int sib) {
-  using C = cpx<T>;
-  extern __shared__ __align__(16) unsigned char rag_smem[];
-  const RClass& c = cls[rag_class(cls, ncls)];
-  const int n = c.nact * c.wgs;
-  int local = (int)blockIdx.x - c.wg0;
-  if (local >= n) return;  // padding up to the next class
-  local = xcd_tile(local, n, c.lgc);
-  const int ii = local / c.wgs, tile = local - ii * c.wgs;
-  const int slot = c.bt.slots[ii];
-  if (c.budget && sib >= c.budget[slot]) return;
-  C* work = (C*)c.a + (long long)slot * c.bt.stride;
-  C* psi = (C*)c.b + (long long)slot * c.bt.stride;
-  double* part = c.partial + (long long)slot * c.bt.pstride;
-  const Phys& ph = c.bt.ia[slot].ph;
-  C* lds = reinterpret_cast<C*>(rag_smem);
-  XPFC_RLSWITCH(c.L, {
-    constexpr int TC = rag_tile(LL);
-    if constexpr (tile_ok<T, LL, TC>()) {
-      xstep_body<T, LL, TC, SAMPLE, LUT>(work, psi, c.segs, tile, (const C*)c.tw, c.g, ph, lds,
-                                         lds + LL * (TC + PADT), part, tile);
-    }
-  })
-}
-
-template <typename T, int DIR>
-__global__ __launch_bounds__(RAG_NT) void k_strided_r(const RClass* __restrict__ cls, int ncls, int sib) {
-  using C = cpx<T>;
-  extern __shared__ __align__(16) unsigned char rag_smem[];
-  const RClass& c = cls[rag_class(cls, ncls)];
-  const int n = c.nact * c.wgs;
-  int local = (int)blockIdx.x - c.wg0;
-  if (local >= n) return;  // padding up to the next class
-  local = xcd_tile(local, n, c.lgc);
-  const int ii = local / c.wgs, tile = local - ii * c.wgs;
-  const int slot = c.bt.slots[ii];
-  if (c.budget && sib >= c.budget[slot]) return;
-  C* buf = (C*)c.a + (long long)slot * c.bt.stride;
-  C* lds = reinterpret_cast<C*>(rag_smem);
-  XPFC_RLSWITCH(c.L, {
-    constexpr int TC = rag_tile(LL);
-    if constexpr (tile_ok<T, LL, TC>())
-      strided_body<T, LL, TC, DIR, false>(buf, c.segs, tile, (const C*)c.tw, lds, lds + LL * (TC + PADT));
-  })
-}
-
-// No occupancy hint: the bodies of every line length share one register
-// allocation, which a 4-waves/SIMD bound would make spill.
-template <typename T, bool SAMPLE, bool PW>
-__global__ __launch_bounds__(RAG_NTZ)
-void k_zpass_r(const RClass* __restrict__ cls, int ncls, int sib) {
-  using C = cpx<T>;
-  extern __shared__ __align__(16) unsigned char rag_smem[];
-  const RClass& c = cls[rag_class(cls, ncls)];
-  const int local = (int)blockIdx.x - c.wg0;
-  const int ii = local / c.wgs, wb = local - ii * c.wgs;
-  if (ii >= c.nact) return;  // padding up to the next class
-  const int slot = c.bt.slots[ii];
-  if (c.budget && sib >= c.budget[slot]) return;
-  C* base = (C*)c.a + (long long)slot * c.bt.stride;
-  C* ny = base + c.nyoff;
-  double* part = c.partial + (long long)slot * c.bt.pzstride;
-  C* lds = reinterpret_cast<C*>(rag_smem);
-  // H = n2 / 2 <= 512 on every supported grid
-#define XPFC_RZ(HH)                                                                               \
-  case HH:                                                                                        \
-    if constexpr (z_ok<T, HH, RAG_NTZ>())                                                         \
-      zpass_body<T, HH, RAG_NTZ, Z_STEP, SAMPLE, PW>(base, ny, base, ny, c.nlines, c.zl,           \
-                                                      c.bt.ia[slot].za, (const C*)c.tw,           \
-                                                      (const C*)c.twp, part, lds, wb);            \
-    break;
-  switch (c.L) {
-    XPFC_RZ(4) XPFC_RZ(8) XPFC_RZ(16) XPFC_RZ(32) XPFC_RZ(64) XPFC_RZ(128) XPFC_RZ(256) XPFC_RZ(512)
-    default: break;
-  }
-#undef XPFC_RZ
-}
-
 namespace {
-// dynamic LDS beyond the default limit needs the per-kernel attribute
-template <class K>
-bool allow_lds(K kernel, size_t bytes) {
-  if (bytes <= 65536) return true;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
-                             (int)bytes) == hipSuccess;
+// F(std::integral_constant<int, FAM>) for fam in {1, 3, 5}
+template <class F>
+int by_family(int fam, F&& f) {
+  switch (fam) {
+    case 1: return f(std::integral_constant<int, 1>{});
+    case 3: return f(std::integral_constant<int, 3>{});
+    case 5: return f(std::integral_constant<int, 5>{});
+    default: return -3;
+  }
 }
-template <typename T>
-int rag_prepare_T(size_t lx, size_t ly, size_t lz) {
-  const bool ok = allow_lds(k_xstep_r<T, true, true>, lx) && allow_lds(k_xstep_r<T, false, true>, lx) &&
-                  allow_lds(k_xstep_r<T, true, false>, lx) && allow_lds(k_xstep_r<T, false, false>, lx) &&
-                  allow_lds(k_strided_r<T, -1>, ly) && allow_lds(k_strided_r<T, 1>, ly) &&
-                  allow_lds(k_zpass_r<T, true, true>, lz) && allow_lds(k_zpass_r<T, true, false>, lz) &&
-                  allow_lds(k_zpass_r<T, false, true>, lz) && allow_lds(k_zpass_r<T, false, false>, lz);
-  return ok ? 0 : launch_status(-2);
-}
+// the family of a real-space half line: H = n2 / 2 of a 2^k, 3*2^k or 5*2^k side
+int half_family(int H) { return line_family(H); }
 }  // namespace
 
-int rag_prepare(int prec, size_t lx, size_t ly, size_t lz) {
-  return prec == 1 ? rag_prepare_T<float>(lx, ly, lz) : rag_prepare_T<double>(lx, ly, lz);
+int rag_prepare(int prec, int fam) {
+  return by_family(fam, [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    return prec == 1 ? RagOps<float, F>::prepare() : RagOps<double, F>::prepare();
+  });
 }
 
-template <typename T, bool LUT>
-static int xr_T(const RClass* cls, int ncls, int nwg, size_t lds, bool sample, int sib, hipStream_t s) {
-  if (nwg <= 0) return 0;
-  if (sample)
-    hipLaunchKernelGGL((k_xstep_r<T, true, LUT>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
-  else
-    hipLaunchKernelGGL((k_xstep_r<T, false, LUT>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
-  return launch_status();
+int launch_xstep_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, bool lut,
+                        bool sample, int sib, hipStream_t s) {
+  return by_family(fam, [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    return prec == 1 ? RagOps<float, F>::xstep(cls, ncls, nwg, lds, lut, sample, sib, s)
+                     : RagOps<double, F>::xstep(cls, ncls, nwg, lds, lut, sample, sib, s);
+  });
 }
 
-int launch_xstep_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool lut, bool sample,
-                        int sib, hipStream_t s) {
-  if (prec == 1)
-    return lut ? xr_T<float, true>(cls, ncls, nwg, lds, sample, sib, s)
-               : xr_T<float, false>(cls, ncls, nwg, lds, sample, sib, s);
-  return lut ? xr_T<double, true>(cls, ncls, nwg, lds, sample, sib, s)
-             : xr_T<double, false>(cls, ncls, nwg, lds, sample, sib, s);
+int launch_strided_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, int dir,
+                          int sib, hipStream_t s) {
+  return by_family(fam, [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    return prec == 1 ? RagOps<float, F>::strided(cls, ncls, nwg, lds, dir, sib, s)
+                     : RagOps<double, F>::strided(cls, ncls, nwg, lds, dir, sib, s);
+  });
 }
 
-template <typename T>
-static int yr_T(const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib, hipStream_t s) {
-  if (nwg <= 0) return 0;
-  if (dir < 0)
-    hipLaunchKernelGGL((k_strided_r<T, -1>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
-  else
-    hipLaunchKernelGGL((k_strided_r<T, 1>), dim3(nwg), dim3(RAG_NT), lds, s, cls, ncls, sib);
-  return launch_status();
+int launch_zpass_ragged(int prec, int fam, const RClass* cls, int ncls, int nwg, size_t lds, bool sample,
+                        bool pw, int sib, hipStream_t s) {
+  return by_family(fam, [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    return prec == 1 ? RagOps<float, F>::zpass(cls, ncls, nwg, lds, sample, pw, sib, s)
+                     : RagOps<double, F>::zpass(cls, ncls, nwg, lds, sample, pw, sib, s);
+  });
 }
 
-int launch_strided_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, int dir, int sib,
-                          hipStream_t s) {
-  return prec == 1 ? yr_T<float>(cls, ncls, nwg, lds, dir, sib, s) : yr_T<double>(cls, ncls, nwg, lds, dir, sib, s);
-}
-
-template <typename T, bool SAMPLE, bool PW>
-static int zr_go(const RClass* cls, int ncls, int nwg, size_t lds, int sib, hipStream_t s) {
-  hipLaunchKernelGGL((k_zpass_r<T, SAMPLE, PW>), dim3(nwg), dim3(RAG_NTZ), lds, s, cls, ncls, sib);
-  return launch_status();
-}
-
-template <typename T>
-static int zr_T(const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw, int sib, hipStream_t s) {
-  if (nwg <= 0) return 0;
-  if (sample) return pw ? zr_go<T, true, true>(cls, ncls, nwg, lds, sib, s) : zr_go<T, true, false>(cls, ncls, nwg, lds, sib, s);
-  return pw ? zr_go<T, false, true>(cls, ncls, nwg, lds, sib, s) : zr_go<T, false, false>(cls, ncls, nwg, lds, sib, s);
-}
-
-int launch_zpass_ragged(int prec, const RClass* cls, int ncls, int nwg, size_t lds, bool sample, bool pw,
-                        int sib, hipStream_t s) {
-  return prec == 1 ? zr_T<float>(cls, ncls, nwg, lds, sample, pw, sib, s)
-                   : zr_T<double>(cls, ncls, nwg, lds, sample, pw, sib, s);
-}
-
-template <typename T>
-static int rsw_T(int L, const Segs& segs) {
-  XPFC_RLSWITCH(L, {
-    constexpr int TC = rag_tile(LL);
-    if constexpr (tile_ok<T, LL, TC>()) {
-      const Segs sg = fill_tiles(segs, TC);
-      return sg.tiles0 + sg.tiles1;
-    }
-    return -3;
-  })
-  return -1;
-}
 int rag_strided_wgs(int prec, int L, const Segs& segs) {
-  return prec == 1 ? rsw_T<float>(L, segs) : rsw_T<double>(L, segs);
+  if (L < 8) return -1;
+  return by_family(line_family(L), [&](auto f) {
+    constexpr int F = decltype(f)::value;
+    return prec == 1 ? RagOps<float, F>::strided_wgs(L, segs) : RagOps<double, F>::strided_wgs(L, segs);
+  });
 }
 size_t rag_strided_lds(int prec, int L) {
   if (L < 8) return 0;
   return ((size_t)L * (rag_tile(L) + PADT) + L) * (prec == 1 ? 8 : 16);
 }
 
-// (the half-line lengths k_zpass_r compiles: H = 4 .. 512)
-template <typename T>
-static long long rzw_T(int H, long long nlines, size_t* lds) {
-  switch (H) {
-#define XPFC_RZW(HH)                                                               \
-  case HH:                                                                         \
-    if constexpr (z_ok<T, HH, RAG_NTZ>()) {                                        \
-      constexpr int G = RAG_NTZ / z_tpl<T, HH>();                                  \
-      if (lds) *lds = (size_t)z_lds_elems<T, HH, RAG_NTZ>() * sizeof(cpx<T>);     \
-      return (nlines + G - 1) / G;                                                 \
-    }                                                                              \
-    return -3;
-    XPFC_RZW(4) XPFC_RZW(8) XPFC_RZW(16) XPFC_RZW(32) XPFC_RZW(64) XPFC_RZW(128) XPFC_RZW(256) XPFC_RZW(512)
-#undef XPFC_RZW
-    default: return -1;
-  }
+namespace {
+long long rzw(int prec, int H, long long nlines, size_t* lds) {
+  if (H < 4) return -1;
+  return by_family(half_family(H), [&](auto f) -> int {
+    constexpr int F = decltype(f)::value;
+    const long long n = prec == 1 ? RagOps<float, F>::zpass_wgs(H, nlines, lds)
+                                  : RagOps<double, F>::zpass_wgs(H, nlines, lds);
+    return n > INT_MAX ? -1 : (int)n;
+  });
 }
-int rag_zpass_wgs(int prec, int H, long long nlines) {
-  return (int)(prec == 1 ? rzw_T<float>(H, nlines, nullptr) : rzw_T<double>(H, nlines, nullptr));
-}
+}  // namespace
+int rag_zpass_wgs(int prec, int H, long long nlines) { return (int)rzw(prec, H, nlines, nullptr); }
 size_t rag_zpass_lds(int prec, int H) {
   size_t b = 0;
-  if ((prec == 1 ? rzw_T<float>(H, 1, &b) : rzw_T<double>(H, 1, &b)) < 0) return 0;
+  if (rzw(prec, H, 1, &b) < 0) return 0;
   return b;
 }
 

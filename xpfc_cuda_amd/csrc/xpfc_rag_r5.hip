@@ -1,0 +1,6 @@
+// Ragged (several-grid) step passes over lines of the 5*2^k-point family.
+#include "xpfc_ragged.hpp"
+
+namespace xpfc {
+XPFC_INSTANTIATE_RAG(5)
+}  // namespace xpfc
