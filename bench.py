@@ -44,6 +44,13 @@ except ValueError:
     os.environ["GPU_MAX_HW_QUEUES"] = str(HW_QUEUES)
 
 HBM_PEAK_GBS = 8000.0
+# Algorithmic HBM bytes in spectra S (one packed half spectrum): the axis-0
+# pass reads the nonlinear term and the state and writes the state (3 S); the
+# axis-1 inverse, real-space and axis-1 forward passes read and write one
+# spectrum each (2 S apiece), so a 3-D step moves 9 S (2-D: 5 S).  (With the
+# state kept in k-space the axis-0 pass would also write it back: 4 S, 10 S.)
+XSTEP_SPECTRA = 3
+STEP_SPECTRA_3D = 9
 
 
 def fcc_params(n, prec, t_sample=100):
@@ -389,7 +396,7 @@ def main():
             ctx.get_field()
         print(json.dumps({"traffic_probe": True, "n": args.n, "steps": args.steps,
                           "copy_bytes": spectrum_bytes(args.n, 1),
-                          "xstep_algorithmic_bytes": 4 * spectrum_bytes(args.n, 1)}), flush=True)
+                          "xstep_algorithmic_bytes": XSTEP_SPECTRA * spectrum_bytes(args.n, 1)}), flush=True)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -415,10 +422,10 @@ def main():
     ms_per_step = ms / args.steps
     value = 1000.0 / ms_per_step
     S = spectrum_bytes(n, 1)
-    # dominant kernel: the axis-0 fused pass, 4 S algorithmic bytes per launch
+    # dominant kernel: the axis-0 fused pass, 3 S algorithmic bytes per launch
     x_ms = passes[0]
-    achieved = 4.0 * S / (x_ms * 1e-3) / 1e9
-    step_gbs = 10.0 * S / (passes[4] * 1e-3) / 1e9
+    achieved = XSTEP_SPECTRA * S / (x_ms * 1e-3) / 1e9
+    step_gbs = STEP_SPECTRA_3D * S / (passes[4] * 1e-3) / 1e9
     out = {
         "metric": "XPFC steps/s at 512^3 (FP32-mixed)",
         "value": round(value, 3),
@@ -442,8 +449,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                      "traffic": None,
-                     "kernel": "k_xstep (axis-0 FFT + update + inverse FFT)",
-                     "algorithmic_bytes": 4 * S},
+                     "kernel": "k_xstep (axis-0 FFTs of the nonlinear term and the state, update, inverse FFT)",
+                     "algorithmic_bytes": XSTEP_SPECTRA * S},
         "samples": [s.line().strip() for s in samples[:4]],
     }
     # the practical ceiling: a plain streaming copy of one spectrum (read + write)
@@ -457,7 +464,7 @@ def main():
         v64 = 1000.0 / (ms64 / fsteps)
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
-        out["fp64_step_hbm_gbs"] = round(10.0 * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
+        out["fp64_step_hbm_gbs"] = round(STEP_SPECTRA_3D * spectrum_bytes(n, 0) / (passes64[4] * 1e-3) / 1e9, 1)
         out["accuracy_vs_fp64"] = accuracy_run(n)
     out["variants"] = {"axis0_tile": variants[0], "axis1_tile": variants[1] & ~1,
                        "axis1_xcd_pairs": bool(variants[1] & 1),

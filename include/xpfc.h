@@ -247,7 +247,8 @@ int xpfc_read_vtk(const char* path, int* dims, double* spacing, double* host_rea
  * working directory like the reference. */
 int xpfc_inst_initialize_vtk(xpfc_ctx* ctx, int s, const char* path);
 /* Binary checkpoint of one instance: parameters, step, FP64 mean and the
- * device spectra (the state and the nonlinear-term work field), so a run
+ * device buffers (the state, psi_k transformed back along axis 0, and the
+ * nonlinear-term work field), so a run
  * resumed from it continues bit for bit.  Loading requires the same grid and
  * precision; the sample log is cleared. */
 int xpfc_inst_save_checkpoint(xpfc_ctx* ctx, int s, const char* path);

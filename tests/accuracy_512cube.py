@@ -6,7 +6,10 @@ oracle takes about 25 CPU-minutes with 16 threads.  It runs in two halves:
 
     python tests/accuracy_512cube.py gpu DIR          # on the GPU: writes DIR/gpu.npz
     python tests/accuracy_512cube.py oracle DIR [W]   # on any host: runs the oracle
-                                                      # (W threads), compares, prints JSON
+                                                      # (W threads), writes DIR/oracle.npz,
+                                                      # compares, prints JSON
+    python tests/accuracy_512cube.py compare DIR      # compares DIR/gpu.npz with an
+                                                      # existing DIR/oracle.npz
 
 The GPU half runs the workload (fcc two-peak kernel, sigma 0.1, n0 0.05,
 noise 0.01 from the seed-2 Philox draws, FE/RMSE every 100 steps) for 1000
@@ -14,8 +17,9 @@ steps in FP32-mixed and in FP64 and keeps the free-energy / RMSE / mean
 samples and every 8th point of the field along each axis (64^3 values), so the
 file stays small.  The oracle half starts from the same draws (noise_ic, the
 host form of the device generator), runs the reference's loop with the same
-sampling schedule, and reports max |dn| over the kept points, the relative
-free-energy difference at every sample and the RMSE samples.  Not collected by
+sampling schedule, keeps the same subsample of its field and its samples, and
+reports max |dn| over the kept points, the relative free-energy difference at
+every sample and the RMSE samples.  Not collected by
 pytest (no test_ prefix); the result is summarised under profiles/.
 """
 import json
@@ -64,25 +68,36 @@ def gpu(outdir):
 
 def oracle(outdir, workers):
     from oracle import xpfc_oracle as O
-    g = np.load(os.path.join(outdir, "gpu.npz"))
     k1 = 2.0 * math.pi
     L = N * 0.125
     op = O.Params(n=(N, N, N), L=(L, L, L), sigma=0.1, psi0=0.05,
                   peaks=[O.Peak(k1, 1.0, 1.0), O.Peak(k1 * 2.0 / math.sqrt(3.0), 1.0, 1.0)],
                   dcf_mode="max", t_sample=T_SAMPLE)
     psi0 = O.noise_ic(op, AMP, SEED)
-    init_diff = {k: float(np.max(np.abs(g[k + "_init"] - psi0[::SUB, ::SUB, ::SUB]))) for k in ("fp32", "fp64")}
+    init = psi0[::SUB, ::SUB, ::SUB].copy()
     t0 = time.perf_counter()
     ref, rs = O.run(psi0, op, STEPS, sample=True, workers=workers)
     dt = time.perf_counter() - t0
     del psi0
-    ofe = {s.step: s.value for s in rs if s.kind == "FE"}
-    osse = {s.step: s.value for s in rs if s.kind == "SSE"}
-    sub = ref[::SUB, ::SUB, ::SUB]
+    np.savez_compressed(os.path.join(outdir, "oracle.npz"), init=init, field=ref[::SUB, ::SUB, ::SUB].copy(),
+                        mean=np.array(float(np.mean(ref))),
+                        fe=np.array([(s.step, s.value) for s in rs if s.kind == "FE"]),
+                        sse=np.array([(s.step, s.value) for s in rs if s.kind == "SSE"]),
+                        seconds=np.array(dt), threads=np.array(workers))
+    compare(outdir)
+
+
+def compare(outdir):
+    g = np.load(os.path.join(outdir, "gpu.npz"))
+    o = np.load(os.path.join(outdir, "oracle.npz"))
+    init_diff = {k: float(np.max(np.abs(g[k + "_init"] - o["init"]))) for k in ("fp32", "fp64")}
+    ofe = {int(s): v for s, v in o["fe"]}
+    osse = {int(s): v for s, v in o["sse"]}
+    sub = o["field"]
     res = {"workload": "3-D fcc two-peak 512^3, sigma 0.1, n0 0.05, noise 0.01 seed 2, %d steps" % STEPS,
-           "oracle_seconds": round(dt, 1), "oracle_threads": workers,
+           "oracle_seconds": round(float(o["seconds"]), 1), "oracle_threads": int(o["threads"]),
            "points_compared": int(sub.size), "initial_field_max_abs_diff": init_diff,
-           "oracle_mean": float(np.mean(ref))}
+           "oracle_mean": float(o["mean"])}
     for k in ("fp32", "fp64"):
         fe = {int(s): v for s, v in g[k + "_fe"]}
         sse = {int(s): v for s, v in g[k + "_sse"]}
@@ -99,9 +114,11 @@ def oracle(outdir, workers):
 
 
 if __name__ == "__main__":
-    if len(sys.argv) < 3 or sys.argv[1] not in ("gpu", "oracle"):
+    if len(sys.argv) < 3 or sys.argv[1] not in ("gpu", "oracle", "compare"):
         sys.exit(__doc__)
     if sys.argv[1] == "gpu":
         gpu(sys.argv[2])
+    elif sys.argv[1] == "compare":
+        compare(sys.argv[2])
     else:
         oracle(sys.argv[2], int(sys.argv[3]) if len(sys.argv) > 3 else (os.cpu_count() or 1))

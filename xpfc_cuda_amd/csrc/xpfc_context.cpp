@@ -325,6 +325,20 @@ int check_inst(const xpfc_ctx* c, int s) {
 namespace {
 // A context of K instances of one grid on its own stream, or (shared != null)
 // on the stream of the mixed-grid context it is a class of.
+// The state form of a context (KGeom::xreal).  Held along axis 0 in real
+// space, the axis-0 pass writes one spectrum less per step and transforms the
+// state once more: measured on MI355X, that pays for 3-D spectra of 16 MiB and
+// more (256^3 FP32 +6 %, 512^3 FP32 +8 %, 128^3 FP64 +6 %, 512^3 FP64 +10 %)
+// and costs below (128^3 FP32 -2 %, 64^3 FP32 -1 %) and for every 2-D grid,
+// whose step has only two passes (256^2 to 2048^2: -5 to -13 %).
+// XPFC_STATE_FORM=0 (psi_k) / 1 (axis-0 form) overrides the choice.
+int state_form(const xpfc_ctx* c) {
+  const char* e = std::getenv("XPFC_STATE_FORM");
+  if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] - '0';
+  const size_t sbytes = (size_t)c->stride * c->csize;
+  return (c->n1 > 1 && sbytes >= ((size_t)16 << 20)) ? 1 : 0;
+}
+
 int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out) {
   if (!ps || !out || K < 1) return fail(XPFC_ERR_ARG, "bad arguments");
   *out = nullptr;
@@ -452,6 +466,8 @@ int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out
   c->g.dk2 = 2.0 * M_PI / L2;
   c->g.cubic = (c->g.dk0 == c->g.dk2 && (nd == 2 || c->g.dk1 == c->g.dk0)) ? 1 : 0;
   c->g.dksq = c->g.dk0 * c->g.dk0;
+  c->g.inv_n0 = 1.0 / (double)c->n0;
+  c->g.xreal = state_form(c);
 
   c->inst.resize(K);
   std::vector<InstArgs> ia(K + 1);
@@ -620,12 +636,13 @@ Segs x_segs(const xpfc_ctx* c, int tc) {
 // axis-1 pass with variant v: tile width and workgroup order.  The streaming
 // hints apply to tiles whose rows are whole 128-byte lines (measured at 512^3
 // FP32: with 32- and 64-byte rows they double the pass's HBM traffic).
-int launch_y(xpfc_ctx* c, void* spec, int v, int dir, const Batch& bt, int ny) {
+// `in`: the pass's input when it is not `spec` itself.
+int launch_y(xpfc_ctx* c, void* spec, int v, int dir, const Batch& bt, int ny, const void* in = nullptr) {
   Segs sg = c->segY;
   sg.lgc = (v & YRUNS_PAIRED) ? XCD_RUN_Y : -1;
   const int tile = v & ~YRUNS_PAIRED;
   const bool nt = c->nt_y && (size_t)tile * c->csize >= 128;
-  return launch_strided(c->prec, spec, sg, c->n1, tile, dir, c->tw1, bt, ny, nt, c->stream);
+  return launch_strided(c->prec, in ? in : spec, spec, sg, c->n1, tile, dir, c->tw1, bt, ny, nt, c->stream);
 }
 
 int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
@@ -635,20 +652,21 @@ int forward_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x) {
   return XPFC_OK;
 }
 
-int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny) {
-  LCHK(launch_strided(c->prec, spec, x_segs(c, c->tc_x), c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
+int inverse_xy(xpfc_ctx* c, void* spec, const Batch& bt, int ny, bool do_x = true) {
+  if (do_x)
+    LCHK(launch_strided(c->prec, spec, x_segs(c, c->tc_x), c->n0, c->tc_x, +1, c->tw0, bt, ny, false, c->stream));
   if (c->n1 > 1) LCHK(launch_y(c, spec, c->tc_y, +1, bt, ny));
   return XPFC_OK;
 }
 
-// instance s: psi state spectrum -> real field (working precision) in its aux
-// region, optional real-space sums into its Z partials
+// instance s: the state -> real field (working precision) in its aux region,
+// optional real-space sums into its Z partials
 int psi_to_real(xpfc_ctx* c, int s, bool sums) {
   const size_t sbytes = (size_t)c->stride * c->csize;
   HIPCHK(hipMemcpyAsync(inst_ptr(c, c->aux, s), inst_ptr(c, c->psi, s), sbytes,
                         hipMemcpyDeviceToDevice, c->stream));
   const Batch bt = bt_one(c, s);
-  int rc = inverse_xy(c, c->aux, bt, 1);
+  int rc = inverse_xy(c, c->aux, bt, 1, !c->g.xreal);
   if (rc) return rc;
   LCHK(launch_zpass(c->prec, Z_C2R, c->aux, ny_of(c, c->aux), c->aux, nullptr,
                     (long long)c->n0 * c->n1, c->H, zl_own(c), 256, bt, 1, c->twz, c->twp, c->pz,
@@ -711,11 +729,15 @@ int field_from_aux(xpfc_ctx* c, int s) {
   void* aux = inst_ptr(c, c->aux, s);
   void* psi = inst_ptr(c, c->psi, s);
   void* work = inst_ptr(c, c->work, s);
-  // psi_k = FFT(psi - mean): the k = 0 slot is then zero by construction
+  // psi_k = FFT(psi - mean) with the k = 0 slot set to zero; in the axis-0
+  // state form, transformed back along axis 0
   LCHK(launch_zpass(c->prec, Z_R2C_FLUCT, aux, nullptr, psi, ny_of(c, psi), nl, c->H, zl_own(c), 256,
                     here, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
   if ((rc = forward_xy(c, c->psi, bt, 1, true))) return rc;
   HIPCHK(hipMemsetAsync(psi, 0, c->csize, c->stream));
+  if (c->g.xreal)
+    LCHK(launch_strided(c->prec, c->psi, x_segs(c, c->tc_x), c->n0, c->tc_x, +1, c->tw0, bt, 1, false,
+                        c->stream));
   // work = FFT_yz(N(psi)): the nonlinear term, all axes but axis 0
   LCHK(launch_zpass(c->prec, Z_R2C_NL, aux, nullptr, work, ny_of(c, work), nl, c->H, zl_own(c), 256,
                     here, 1, c->twz, c->twp, nullptr, false, c->any_powf, c->stream));
@@ -937,9 +959,12 @@ int launch_one_step(xpfc_ctx* c, bool sample, int row, hipEvent_t* ev = nullptr,
                                c->px, sample, c->stream);
   LCHK(nbx);
   if (ev) HIPCHK(hipEventRecord(ev[0], c->stream));
-  if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, +1, bt, ny));
+  // in the axis-0 state form the state itself feeds the next pass (axis 1 in
+  // 3-D, real space in 2-D), which writes the working buffer
+  if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, +1, bt, ny, c->g.xreal ? c->psi : nullptr));
   if (ev) HIPCHK(hipEventRecord(ev[1], c->stream));
-  LCHK(launch_zpass(c->prec, Z_STEP, c->work, ny_of(c, c->work), c->work, ny_of(c, c->work), nl, c->H,
+  void* zin = (c->n1 > 1 || !c->g.xreal) ? c->work : c->psi;
+  LCHK(launch_zpass(c->prec, Z_STEP, zin, ny_of(c, zin), c->work, ny_of(c, c->work), nl, c->H,
                     zl_own(c), c->ntz, bt, ny, c->twz, c->twp, c->pz, sample, c->any_powf, c->stream));
   if (ev) HIPCHK(hipEventRecord(ev[2], c->stream));
   if (c->n1 > 1) LCHK(launch_y(c, c->work, c->tc_y, -1, bt, ny));
@@ -1155,7 +1180,20 @@ int initial_records(xpfc_ctx* c) {
 int fe_now(xpfc_ctx* c, int s, double* F, double* fm, double* mean_rs) {
   int nb = 0;
   const Batch bt = bt_one(c, s);
-  LCHK(launch_kspace_sums(c->prec, c->psi, c->nmain, c->nny, c->g, bt, c->px, &nb, c->stream));
+  // psi_k: the state, or in the axis-0 form psi_k as the axis-0 pass forms
+  // it (FFT_0(state) / n0, k = 0 slot zero) in the aux region
+  void* spec = c->psi;
+  if (c->g.xreal) {
+    void* aux = inst_ptr(c, c->aux, s);
+    HIPCHK(hipMemcpyAsync(aux, inst_ptr(c, c->psi, s), (size_t)c->stride * c->csize, hipMemcpyDeviceToDevice,
+                          c->stream));
+    LCHK(launch_strided(c->prec, c->aux, x_segs(c, c->tc_x), c->n0, c->tc_x, -1, c->tw0, bt, 1, false,
+                        c->stream));
+    LCHK(launch_scale(c->prec, aux, 2LL * c->stride, c->g.inv_n0, c->stream));
+    HIPCHK(hipMemsetAsync(aux, 0, c->csize, c->stream));
+    spec = c->aux;
+  }
+  LCHK(launch_kspace_sums(c->prec, spec, c->nmain, c->nny, c->g, bt, c->px, &nb, c->stream));
   int rc = psi_to_real(c, s, true);
   if (rc) return rc;
   double* row = c->sums + (size_t)DIAG_ROW * c->K * NSUMS;
@@ -1778,15 +1816,17 @@ extern "C" int xpfc_inst_write_vtk(xpfc_ctx* c, int s, const char* path) {
 extern "C" int xpfc_write_vtk(xpfc_ctx* c, const char* path) { return xpfc_inst_write_vtk(c, 0, path); }
 
 // ---------------------------------------------------------------------------
-// binary checkpoints: header, then the state spectrum and the work spectrum of
-// the instance exactly as they sit in device memory (packed half spectrum)
+// binary checkpoints: header, then the state (in the context's form) and the
+// work spectrum of the instance exactly as they sit in device memory (packed
+// half-spectrum layout)
 // ---------------------------------------------------------------------------
 namespace {
 struct CkptHeader {
   char magic[8];          // "XPFCCKP1"
-  int32_t version;        // 1
+  int32_t version;        // 3
   int32_t precision;
   int32_t n0, n1, n2;     // packed-layout extents
+  int32_t state_form;     // KGeom::xreal of the writing context
   int64_t step;
   double mean;
   int64_t stride;         // complex elements per spectrum
@@ -1805,7 +1845,8 @@ extern "C" int xpfc_inst_save_checkpoint(xpfc_ctx* c, int s, const char* path) {
   const XInst& in = c->inst[s];
   CkptHeader h{};
   std::memcpy(h.magic, kMagic, 8);
-  h.version = 2;  // 2: xpfc_params with powf_fp32
+  h.version = 3;  // 2: xpfc_params with powf_fp32; 3: the state form
+  h.state_form = c->g.xreal;
   h.precision = c->prec;
   h.n0 = c->n0;
   h.n1 = c->n1;
@@ -1839,7 +1880,7 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
   const size_t bytes = (size_t)c->stride * c->csize;
   std::vector<char> buf(2 * bytes);
   bool ok = std::fread(&h, sizeof(h), 1, f) == 1;
-  ok = ok && std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 2;
+  ok = ok && std::memcmp(h.magic, kMagic, 8) == 0 && h.version == 3;
   if (!ok) {
     std::fclose(f);
     return fail(XPFC_ERR_PARSE, "%s: not an XPFC checkpoint", path);
@@ -1852,6 +1893,10 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
       h.stride != c->stride || h.csize != (int64_t)c->csize || !same_grid(c->p, h.params)) {
     std::fclose(f);
     return fail(XPFC_ERR_ARG, "%s: grid, box or precision differ from the context's", path);
+  }
+  if (h.state_form != c->g.xreal) {
+    std::fclose(f);
+    return fail(XPFC_ERR_ARG, "%s: the state is held in the other form (XPFC_STATE_FORM differs)", path);
   }
   ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
   std::fclose(f);
@@ -2002,6 +2047,7 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
       y.segs = filled(q->segY, rag_tile(q->n1));
       y.bt = bt;
       y.a = q->work;
+      y.b = q->g.xreal ? q->psi : q->work;  // the inverse pass reads the state in the axis-0 form
       y.tw = q->tw1;
       y.budget = r.budget;
       Y.push_back({k, y});
@@ -2013,6 +2059,7 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     z.pw = q->any_powf;
     z.bt = bt;
     z.a = q->work;
+    z.b = (q->n1 > 1 || !q->g.xreal) ? q->work : q->psi;  // 2-D axis-0 form: reads the state
     z.tw = q->twz;
     z.twp = q->twp;
     z.partial = q->pz;

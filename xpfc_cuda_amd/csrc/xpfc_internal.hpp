@@ -37,6 +37,10 @@ struct KGeom {
   // integer m = i^2 + j^2 + l^2, and C2 comes from a per-instance table over m.
   int cubic;
   double dksq;
+  // State form: 0 = psi_k; 1 = psi_k transformed back along axis 0 (then
+  // psi_k = FFT_0(state) * inv_n0, and the axis-0 pass does not write psi_k).
+  int xreal;
+  double inv_n0;  // 1 / n0
 };
 
 // Physics constants of the semi-implicit update (all FP64).
@@ -137,8 +141,9 @@ struct RClass {
   Segs segs;        // strided passes: the class's segments, tiles filled
   KGeom g;          // axis-0 pass
   Batch bt;         // slots (the class's active list), instance table, strides
-  void* a;          // axis-0: work; axis-1 and real space: the buffer (in place)
-  void* b;          // axis-0: psi
+  void* a;          // axis-0: work; axis-1 and real space: the output (work)
+  void* b;          // axis-0: psi (the state); axis-1: the inverse pass's input;
+                    // real space: the input (psi only for 2-D grids in the axis-0 state form)
   const void* tw;   // line twiddles
   const void* twp;  // real space: half-spectrum twiddles
   double* partial;  // sampled partial sums (X or Z) of the class
@@ -188,9 +193,14 @@ int launch_status(int ok_value = 0);
 hipError_t launch_error();
 
 // `ny` is the number of instances covered (entries of bt.slots used).  `nt`
-// selects the streaming-hint loads and stores of the strided pass.
-int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
+// selects the streaming-hint loads and stores of the strided pass, which
+// reads `in` and writes `buf` (the same buffer, or two of one layout).
+int launch_strided(int prec, const void* in, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
                    const Batch& bt, int ny, bool nt, hipStream_t s);
+inline int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
+                          const Batch& bt, int ny, bool nt, hipStream_t s) {
+  return launch_strided(prec, buf, buf, segs, L, tc, dir, tw, bt, ny, nt, s);
+}
 int launch_xstep(int prec, void* work, void* psi, const Segs& segs, int L, int tc, const void* tw,
                  const KGeom& g, const Batch& bt, int ny, double* partial, bool sample,
                  hipStream_t s);

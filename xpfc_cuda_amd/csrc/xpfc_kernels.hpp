@@ -5,18 +5,31 @@
 // Spectrum layout (packed half spectrum): for a real field of n0 x n1 x n2
 // points (n2 contiguous; 2-D fields use n1 = 1) the spectrum is
 //   main[n0][n1][H]  (kz = 0 .. H-1, H = n2/2)  followed by  ny[n0][n1] (kz = H).
-// The k = 0 slot of `main` always holds 0: the mean density (the k = 0 mode)
-// is carried separately in FP64 and added back in real space.
+// The k = 0 mode is always 0: the mean density is carried separately in FP64
+// and added back in real space.
 //
-// One time step = four passes over HBM (two in 2-D):
-//   k_xstep   : lines along axis 0: forward FFT of the nonlinear term, the
-//               semi-implicit update of psi_k, inverse FFT      (euler_step.cu:50-59)
-//   k_strided : lines along axis 1, inverse                      (3-D only)
+// The state between steps, in the `psi` buffer, takes one of two forms (per
+// context, KGeom::xreal): psi_k itself, or psi_k transformed back along axis 0
+// (the unnormalised inverse transform over kx; spectral along the other axes).
+// `work` carries the nonlinear term N_k transformed along every axis but
+// axis 0.  One time step = four passes over HBM (two in 2-D):
+//   k_xstep   : lines along axis 0: forward FFT of the nonlinear term (and of
+//               the state in the second form: psi_k = FFT_0(state) / n0), the
+//               semi-implicit update of psi_k, inverse FFT
+//                                                             (euler_step.cu:50-59)
+//               k-space form: psi_k' and its inverse transform (into work)
+//               are written, 4 spectra of traffic; axis-0 form: only the
+//               inverse transform, which is the new state, 3 spectra
+//   k_strided : lines along axis 1, inverse (axis-0 form: state -> work)
+//                                                                 (3-D only)
 //   k_zpass   : lines along axis 2: C2R, 1/N, N(psi) = -a psi^2 + b psi^3,
-//               R2C                                               (power_functions.cu:34-53,
+//               R2C (2-D axis-0 form: state -> work)      (power_functions.cu:34-53,
 //                                                                  scaling_step.cu:48-49,
 //                                                                  copy_matrix.cu:44-45)
 //   k_strided : lines along axis 1, forward                      (3-D only)
+// The axis-0 form saves one spectrum of traffic per step at the price of one
+// more transform of the state; it pays where the spectra stream from HBM and
+// costs where they are cache-resident (small grids), hence the choice.
 // The sampled diagnostics (FE_calc.cu:64, sse_calculation.cu:133-140,
 // Avg_calc.cu:3-18) are FP64 partial sums fused into k_xstep and k_zpass.
 //
@@ -116,19 +129,22 @@ __device__ __forceinline__ TileLoc locate(const Segs& segs, int tile, int ci, in
 // +7 %; the step -2.3 % in FP32, -4 % in FP64); for fields that fit the cache
 // the hint costs up to 15 %.
 // One column tile of a strided pass (buf already at the instance's spectra).
+// `in` and `out` are the same buffer (in place) or two buffers of one layout:
+// every line is read before the transform and written after it.
 template <typename T, int L, int TC, int DIR, bool NT>
-__device__ __forceinline__ void strided_body(cpx<T>* __restrict__ buf, const Segs& segs, int tile,
+__device__ __forceinline__ void strided_body(const cpx<T>* in, cpx<T>* out, const Segs& segs, int tile,
                                              const cpx<T>* __restrict__ tw, cpx<T>* lds, cpx<T>* twl) {
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, tile, ci, TC);
   const int rs = loc.rs;
-  C* p = buf + loc.off;
+  const C* src = in + loc.off;
+  C* p = out + loc.off;
   C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    C* q = p + (t + m * (L / E)) * rs;
+    const C* q = src + (t + m * (L / E)) * rs;
     v[m] = loc.act ? (NT ? load_nt(q) : *q) : mk<C>(0, 0);
   }
   stage_table<tile_threads<L, TC>()>(twl, tw, L);  // read after the first exchange's barrier
@@ -145,18 +161,20 @@ __device__ __forceinline__ void strided_body(cpx<T>* __restrict__ buf, const Seg
 
 template <typename T, int L, int TC, int DIR, bool NT>
 __global__ __launch_bounds__((tile_threads<L, TC>()))
-void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
+void k_strided(const cpx<T>* in, cpx<T>* out, Segs segs, const cpx<T>* __restrict__ tw, Batch bt) {
   using C = cpx<T>;
   __shared__ C lds[L * (TC + PADT)];
   __shared__ C twl[L];
   const int slot = bt.slots[blockIdx.y];
   if (bt.budget && bt.sib >= bt.budget[slot]) return;  // the whole workgroup
-  buf += (long long)slot * bt.stride;
-  strided_body<T, L, TC, DIR, NT>(buf, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), tw, lds, twl);
+  const long long so = (long long)slot * bt.stride;
+  strided_body<T, L, TC, DIR, NT>(in + so, out + so, segs, xcd_tile(blockIdx.x, gridDim.x, segs.lgc), tw, lds,
+                                  twl);
 }
 
 // ---------------------------------------------------------------------------
-// fused axis-0 pass: FFT(N) along axis 0, semi-implicit update, inverse FFT
+// fused axis-0 pass: FFT(N) and FFT(state) along axis 0, semi-implicit
+// update, inverse FFT into the state
 // ---------------------------------------------------------------------------
 // LUT: square / cubic grid, k^2 = dksq * m from the integer m = i^2 + j^2 + l^2
 // and C2 from the instance's table over m (no FP64 exp in the kernel, which
@@ -170,7 +188,9 @@ void k_strided(cpx<T>* __restrict__ buf, Segs segs, const cpx<T>* __restrict__ t
 // walking 2 or 4 tiles per workgroup with the next tile's loads in flight
 // needs 167-178 VGPRs; fetching the state or C2 after the forward transform
 // is 4 % slower.  The variant below (one tile, both streams and C2 loaded up
-// front) is the fastest measured.
+// front) is the fastest measured.  (Measured with the k-space state, written
+// back beside the working buffer; the state is now held along axis 0 in real
+// space, which drops that write.)
 
 // One tile's loads: the nonlinear term and the state along the column.
 template <typename T, int L, int E>
@@ -206,7 +226,6 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   const TileLoc& loc = x.loc;
   const int rs = loc.rs;
   const bool act = loc.act;
-  C* pw = work + loc.off;
   C* pp = psi + loc.off;
 
   // wavenumber of this column
@@ -227,7 +246,16 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
     }
   }
   C (&v)[E] = x.v;
+  // the nonlinear term's spectrum, and in the axis-0 form the state's
+  // (psi_k = FFT_0(state) / n0, the k = 0 mode exactly 0: the mean is carried
+  // apart).  Transforming the two together through two LDS tiles (one barrier
+  // pair per stage for both) was measured no faster in FP32 and 12 % slower in
+  // FP64 at 512^3.
+  const bool xreal = g.xreal != 0;
   fft_line<T, L, E, -1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
+  if (xreal) fft_line<T, L, E, -1>(x.ps, t, lds, TileIdx<TC, PADT>{ci}, twl);
+  const T s0 = (T)g.inv_n0;
+  const bool kzero = xreal && loc.kind == 0 && loc.col == 0;
 
   const double ky = g.dk1 * (double)jb;
   const double kzv = g.dk2 * (double)kz;
@@ -243,7 +271,7 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   for (int m = 0; m < E; ++m) {
     __builtin_amdgcn_sched_barrier(0);
     const int ka = t + m * (L / E);
-    const C old = x.ps[m];
+    const C old = !xreal ? x.ps[m] : (kzero && ka == 0) ? mk<C>(0, 0) : mk<C>(x.ps[m].x * s0, x.ps[m].y * s0);
     const int ia = signed_index(ka, g.n0);
     double k2, c2;
     if constexpr (LUT) {
@@ -277,15 +305,18 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
     v[m] = nv;
   }
   __builtin_amdgcn_sched_barrier(0);
-  if (act) {
+  if (!xreal && act) {
 #pragma unroll
     for (int m = 0; m < E; ++m) pp[(t + m * (L / E)) * rs] = v[m];
   }
 
+  // back along axis 0 (unnormalised): the new state in the axis-0 form, the
+  // working buffer's input of the next passes in the k-space form
   fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
+  C* out = xreal ? pp : work + loc.off;
   if (act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) pw[(t + m * (L / E)) * rs] = v[m];
+    for (int m = 0; m < E; ++m) out[(t + m * (L / E)) * rs] = v[m];
   }
 }
 

@@ -21,8 +21,8 @@ namespace xpfc {
 // does not support.
 template <typename T, int L>
 struct LenOps {
-  static int strided(void* buf, const Segs& segs, int tc, int dir, const void* tw, const Batch& bt, int ny,
-                     bool nt, hipStream_t s);
+  static int strided(const void* in, void* buf, const Segs& segs, int tc, int dir, const void* tw,
+                     const Batch& bt, int ny, bool nt, hipStream_t s);
   static int blocks(int tc, const Segs& segs);
   static int xstep(void* work, void* psi, const Segs& segs, int tc, const void* tw, const KGeom& g,
                    const Batch& bt, int ny, double* partial, bool sample, hipStream_t s);

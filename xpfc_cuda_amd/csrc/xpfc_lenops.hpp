@@ -24,26 +24,26 @@ namespace xpfc {
   }
 
 template <typename T, int LL, int TT, int DIR>
-void strided_go(void* buf, const Segs& sg, int nb, const void* tw, const Batch& bt, int ny, bool nt,
-                hipStream_t s) {
+void strided_go(const void* in, void* buf, const Segs& sg, int nb, const void* tw, const Batch& bt, int ny,
+                bool nt, hipStream_t s) {
   if (nt)
     hipLaunchKernelGGL((k_strided<T, LL, TT, DIR, true>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0, s,
-                       (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
+                       (const cpx<T>*)in, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
   else
     hipLaunchKernelGGL((k_strided<T, LL, TT, DIR, false>), dim3(nb, ny), dim3(tile_threads<LL, TT>()), 0, s,
-                       (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
+                       (const cpx<T>*)in, (cpx<T>*)buf, sg, (const cpx<T>*)tw, bt);
 }
 
 template <typename T, int LL>
-int LenOps<T, LL>::strided(void* buf, const Segs& segs0, int tc, int dir, const void* tw, const Batch& bt,
-                           int ny, bool nt, hipStream_t s) {
+int LenOps<T, LL>::strided(const void* in, void* buf, const Segs& segs0, int tc, int dir, const void* tw,
+                           const Batch& bt, int ny, bool nt, hipStream_t s) {
   XPFC_TCSWITCH(tc, {
     if constexpr (tile_ok<T, LL, TT>()) {
       const Segs sg = fill_tiles(segs0, TT);
       const int nb = sg.tiles0 + sg.tiles1;
       if (nb == 0 || ny == 0) return 0;
-      if (dir < 0) strided_go<T, LL, TT, -1>(buf, sg, nb, tw, bt, ny, nt, s);
-      else strided_go<T, LL, TT, +1>(buf, sg, nb, tw, bt, ny, nt, s);
+      if (dir < 0) strided_go<T, LL, TT, -1>(in, buf, sg, nb, tw, bt, ny, nt, s);
+      else strided_go<T, LL, TT, +1>(in, buf, sg, nb, tw, bt, ny, nt, s);
     } else {
       return -3;
     }

@@ -99,11 +99,11 @@ hipError_t launch_error() { return t_launch_error; }
 // Each dispatcher returns -1 for a length that is not compiled.
 namespace {
 template <typename T>
-int strided_T(void* buf, const Segs& segs, int L, int tc, int dir, const void* tw, const Batch& bt, int ny,
-              bool nt, hipStream_t s) {
+int strided_T(const void* in, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
+              const Batch& bt, int ny, bool nt, hipStream_t s) {
   switch (L) {
 #define XPFC_C(LEN) \
-  case LEN: return LenOps<T, LEN>::strided(buf, segs, tc, dir, tw, bt, ny, nt, s);
+  case LEN: return LenOps<T, LEN>::strided(in, buf, segs, tc, dir, tw, bt, ny, nt, s);
     XPFC_LENGTHS(XPFC_C)
 #undef XPFC_C
     default: return -1;
@@ -156,10 +156,10 @@ int zblocks_T(int H, int ntz, long long nlines) {
 }
 }  // namespace
 
-int launch_strided(int prec, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
+int launch_strided(int prec, const void* in, void* buf, const Segs& segs, int L, int tc, int dir, const void* tw,
                    const Batch& bt, int ny, bool nt, hipStream_t s) {
-  return prec == 1 ? strided_T<float>(buf, segs, L, tc, dir, tw, bt, ny, nt, s)
-                   : strided_T<double>(buf, segs, L, tc, dir, tw, bt, ny, nt, s);
+  return prec == 1 ? strided_T<float>(in, buf, segs, L, tc, dir, tw, bt, ny, nt, s)
+                   : strided_T<double>(in, buf, segs, L, tc, dir, tw, bt, ny, nt, s);
 }
 
 int strided_blocks(int prec, int L, int tc, const Segs& segs) {

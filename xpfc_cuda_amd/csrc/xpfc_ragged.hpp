@@ -105,13 +105,17 @@ __global__ __launch_bounds__(rag_nt(FAM)) void k_strided_r(const RClass* __restr
   const int ii = local / c.wgs, tile = local - ii * c.wgs;
   const int slot = c.bt.slots[ii];
   if (c.budget && sib >= c.budget[slot]) return;
-  C* buf = (C*)c.a + (long long)slot * c.bt.stride;
+  // the inverse pass reads the state and writes the working buffer; the
+  // forward pass works in place
+  const long long so = (long long)slot * c.bt.stride;
+  C* buf = (C*)c.a + so;
+  const C* in = DIR > 0 ? (const C*)c.b + so : buf;
   C* lds = reinterpret_cast<C*>(rag_smem);
   rag_lines<FAM>(c.L, [&](auto len) {
     constexpr int LL = decltype(len)::value;
     constexpr int TC = rag_tile(LL);
     if constexpr (tile_ok<T, LL, TC>())
-      strided_body<T, LL, TC, DIR, false>(buf, c.segs, tile, (const C*)c.tw, lds, lds + LL * (TC + PADT));
+      strided_body<T, LL, TC, DIR, false>(in, buf, c.segs, tile, (const C*)c.tw, lds, lds + LL * (TC + PADT));
   });
 }
 
@@ -127,15 +131,17 @@ __global__ __launch_bounds__(RAG_NTZ) void k_zpass_r(const RClass* __restrict__ 
   if (ii >= c.nact) return;  // padding up to the next class
   const int slot = c.bt.slots[ii];
   if (c.budget && sib >= c.budget[slot]) return;
-  C* base = (C*)c.a + (long long)slot * c.bt.stride;
-  C* ny = base + c.nyoff;
+  const long long so = (long long)slot * c.bt.stride;
+  C* base = (C*)c.a + so;
+  const C* in = (const C*)c.b + so;
   double* part = c.partial + (long long)slot * c.bt.pzstride;
   C* lds = reinterpret_cast<C*>(rag_smem);
   rag_halves<FAM>(c.L, [&](auto half) {
     constexpr int HH = decltype(half)::value;
     if constexpr (z_ok<T, HH, RAG_NTZ>())
-      zpass_body<T, HH, RAG_NTZ, Z_STEP, SAMPLE, PW>(base, ny, base, ny, c.nlines, c.zl, c.bt.ia[slot].za,
-                                                      (const C*)c.tw, (const C*)c.twp, part, lds, wb);
+      zpass_body<T, HH, RAG_NTZ, Z_STEP, SAMPLE, PW>(in, in + c.nyoff, base, base + c.nyoff, c.nlines, c.zl,
+                                                      c.bt.ia[slot].za, (const C*)c.tw, (const C*)c.twp, part,
+                                                      lds, wb);
   });
 }
 
