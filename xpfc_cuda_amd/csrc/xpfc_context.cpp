@@ -215,13 +215,31 @@ double peak_kcut2(const xpfc_params& p, double ecut) {
   return kc * kc;
 }
 
-// Entries of the C2 table over m = |k|^2 / dk^2 of a square / cubic grid (up
-// to the cutoff, or the largest m on the grid); axis extents (n0, n1, n2).
-int c2_table_entries(double kcut2, double dksq, int n0, int n1, int n2) {
-  const double mmax = (double)(n0 / 2) * (n0 / 2) + (double)(n1 / 2) * (n1 / 2) +
-                      (double)(n2 / 2) * (n2 / 2);
-  return (int)std::min(std::ceil(kcut2 / dksq) + 1.0, mmax + 1.0);
+// The largest m = |k|^2 / dk^2 of a square / cubic grid, axis extents (n0, n1, n2)
+long long grid_mmax(int n0, int n1, int n2) {
+  return (long long)(n0 / 2) * (n0 / 2) + (long long)(n1 / 2) * (n1 / 2) + (long long)(n2 / 2) * (n2 / 2);
 }
+
+// Entries of the C2 table over m of a square / cubic grid (up to the cutoff,
+// or the largest m on the grid).
+int c2_table_entries(double kcut2, double dksq, int n0, int n1, int n2) {
+  return (int)std::min(std::ceil(kcut2 / dksq) + 1.0, (double)grid_mmax(n0, n1, n2) + 1.0);
+}
+
+// The state form (KGeom::xreal) of a grid with n1 points along axis 1 (1 in
+// 2-D) and spectra of `sbytes` bytes.  Held along axis 0 in real space, the
+// axis-0 pass writes one spectrum less per step and transforms the state once
+// more: measured on MI355X, that pays for 3-D spectra of 16 MiB and more
+// (256^3 FP32 +6 %, 512^3 FP32 +8 %, 128^3 FP64 +6 %, 512^3 FP64 +10 %) and
+// costs below (128^3 FP32 -2 %, 64^3 FP32 -1 %) and for every 2-D grid, whose
+// step has only two passes (256^2 to 2048^2: -5 to -13 %).  XPFC_STATE_FORM=0
+// (psi_k) / 1 (axis-0 form) overrides the choice.
+int state_form_of(int n1, size_t sbytes) {
+  const char* e = std::getenv("XPFC_STATE_FORM");
+  if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] - '0';
+  return (n1 > 1 && sbytes >= ((size_t)16 << 20)) ? 1 : 0;
+}
+
 
 // physics of one instance -> its kernel table entry
 int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
@@ -325,20 +343,6 @@ int check_inst(const xpfc_ctx* c, int s) {
 namespace {
 // A context of K instances of one grid on its own stream, or (shared != null)
 // on the stream of the mixed-grid context it is a class of.
-// The state form of a context (KGeom::xreal).  Held along axis 0 in real
-// space, the axis-0 pass writes one spectrum less per step and transforms the
-// state once more: measured on MI355X, that pays for 3-D spectra of 16 MiB and
-// more (256^3 FP32 +6 %, 512^3 FP32 +8 %, 128^3 FP64 +6 %, 512^3 FP64 +10 %)
-// and costs below (128^3 FP32 -2 %, 64^3 FP32 -1 %) and for every 2-D grid,
-// whose step has only two passes (256^2 to 2048^2: -5 to -13 %).
-// XPFC_STATE_FORM=0 (psi_k) / 1 (axis-0 form) overrides the choice.
-int state_form(const xpfc_ctx* c) {
-  const char* e = std::getenv("XPFC_STATE_FORM");
-  if (e && (e[0] == '0' || e[0] == '1') && e[1] == 0) return e[0] - '0';
-  const size_t sbytes = (size_t)c->stride * c->csize;
-  return (c->n1 > 1 && sbytes >= ((size_t)16 << 20)) ? 1 : 0;
-}
-
 int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out) {
   if (!ps || !out || K < 1) return fail(XPFC_ERR_ARG, "bad arguments");
   *out = nullptr;
@@ -467,7 +471,7 @@ int create_impl(const xpfc_params* ps, int K, hipStream_t shared, xpfc_ctx** out
   c->g.cubic = (c->g.dk0 == c->g.dk2 && (nd == 2 || c->g.dk1 == c->g.dk0)) ? 1 : 0;
   c->g.dksq = c->g.dk0 * c->g.dk0;
   c->g.inv_n0 = 1.0 / (double)c->n0;
-  c->g.xreal = state_form(c);
+  c->g.xreal = state_form_of(c->n1, (size_t)c->stride * c->csize);
 
   c->inst.resize(K);
   std::vector<InstArgs> ia(K + 1);

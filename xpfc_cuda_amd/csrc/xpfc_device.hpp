@@ -25,6 +25,19 @@ __device__ __forceinline__ double c2_lookup(int m, const Phys& ph) {
   return m < ph.mcut ? ph.c2tab[m] : 0.0;
 }
 
+// The semi-implicit update's coefficients at one mode (euler_step.cu:50-59):
+// a2 = dt M k^2 and inv = 1 / (1 + a2 (1 - eps) - a2 C2), in FP64 (v_rcp_f64
+// and two Newton steps).
+__device__ __forceinline__ void update_coefs(double k2, double c2, const Phys& ph, double& a2, double& inv) {
+  a2 = k2 * ph.dtM;
+  const double den = 1.0 + a2 * ph.one_m_eps - a2 * c2;  // euler_step.cu:54
+  double r = __builtin_amdgcn_rcp(den);
+  double e = fma(-den, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-den, r, 1.0);
+  inv = fma(r, e, r);
+}
+
 // Complex loads and stores with the non-temporal (streaming) cache hint: the
 // line is not kept in the caches for reuse.
 template <typename C> __device__ __forceinline__ C load_nt(const C* p) {

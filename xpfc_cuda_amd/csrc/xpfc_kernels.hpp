@@ -139,12 +139,14 @@ __device__ __forceinline__ void strided_body(const cpx<T>* in, cpx<T>* out, cons
   const int ci = threadIdx.x % TC, t = threadIdx.x / TC;
   const TileLoc loc = locate(segs, tile, ci, TC);
   const int rs = loc.rs;
-  const C* src = in + loc.off;
-  C* p = out + loc.off;
+  // row t + m * L/E of the column: the thread's offset plus a uniform step
+  const int r0 = t * rs, rstep = (L / E) * rs;  // (< 2^31 for every supported grid)
+  const C* src = in + loc.off + r0;
+  C* p = out + loc.off + r0;
   C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const C* q = src + (t + m * (L / E)) * rs;
+    const C* q = src + m * rstep;
     v[m] = loc.act ? (NT ? load_nt(q) : *q) : mk<C>(0, 0);
   }
   stage_table<tile_threads<L, TC>()>(twl, tw, L);  // read after the first exchange's barrier
@@ -152,7 +154,7 @@ __device__ __forceinline__ void strided_body(const cpx<T>* in, cpx<T>* out, cons
   if (loc.act) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      C* q = p + (t + m * (L / E)) * rs;
+      C* q = p + m * rstep;
       if constexpr (NT) store_nt(q, v[m]);
       else *q = v[m];
     }
@@ -207,13 +209,13 @@ __device__ __forceinline__ void xload(XTile<T, L, elems_per_thread<L>()>& x, con
   constexpr int E = elems_per_thread<L>();
   x.loc = locate(segs, tile < ntiles ? tile : 0, ci, TC);
   x.loc.act = x.loc.act && tile < ntiles;
-  const int rs = x.loc.rs;
-  const C* pw = work + x.loc.off;
-  const C* pp = psi + x.loc.off;
+  const int r0 = t * x.loc.rs, rstep = (L / E) * x.loc.rs;
+  const C* pw = work + x.loc.off + r0;
+  const C* pp = psi + x.loc.off + r0;
 #pragma unroll
-  for (int m = 0; m < E; ++m) x.v[m] = x.loc.act ? pw[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  for (int m = 0; m < E; ++m) x.v[m] = x.loc.act ? pw[m * rstep] : mk<C>(0, 0);
 #pragma unroll
-  for (int m = 0; m < E; ++m) x.ps[m] = x.loc.act ? pp[(t + m * (L / E)) * rs] : mk<C>(0, 0);
+  for (int m = 0; m < E; ++m) x.ps[m] = x.loc.act ? pp[m * rstep] : mk<C>(0, 0);
 }
 
 template <typename T, int L, int TC, bool SAMPLE, bool LUT>
@@ -224,9 +226,9 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   using C = cpx<T>;
   constexpr int E = elems_per_thread<L>();
   const TileLoc& loc = x.loc;
-  const int rs = loc.rs;
+  const int r0 = t * loc.rs, rstep = (L / E) * loc.rs;
   const bool act = loc.act;
-  C* pp = psi + loc.off;
+  C* pp = psi + loc.off + r0;
 
   // wavenumber of this column
   int kb, kz;
@@ -237,12 +239,16 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   // the line's C2 values are gathered with the input streams (gathering them
   // after the forward transform was measured slower, and for 12-point threads
   // it raised the register count)
+  // (A per-m table of the update's FP32 coefficients for every mode, which
+  // would save the FP64 chain, made the pass 22 % slower at 512^3: the C2
+  // table is read only below the cutoff, 9 % of the modes, the full table for
+  // every mode.)
   double c2t[LUT ? E : 1];
   if constexpr (LUT) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int ia = signed_index(t + m * (L / E), g.n0);
-      c2t[m] = c2_lookup(ia * ia + myz, ph);
+      c2t[m] = c2_lookup(__mul24(ia, ia) + myz, ph);  // |ia| <= n0 / 2: 24-bit products
     }
   }
   C (&v)[E] = x.v;
@@ -275,20 +281,15 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
     const int ia = signed_index(ka, g.n0);
     double k2, c2;
     if constexpr (LUT) {
-      k2 = g.dksq * (double)(ia * ia + myz);
+      k2 = g.dksq * (double)(__mul24(ia, ia) + myz);
       c2 = c2t[m];
     } else {
       const double kx = g.dk0 * (double)ia;
       k2 = kx * kx + kyz2;
       c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
     }
-    const double a2 = k2 * ph.dtM;
-    const double den = 1.0 + a2 * ph.one_m_eps - a2 * c2;  // euler_step.cu:54
-    double inv = __builtin_amdgcn_rcp(den);                 // v_rcp_f64 + 2 Newton steps
-    double e = fma(-den, inv, 1.0);
-    inv = fma(inv, e, inv);
-    e = fma(-den, inv, 1.0);
-    inv = fma(inv, e, inv);
+    double a2, inv;
+    update_coefs(k2, c2, ph, a2, inv);
     const C nh = v[m];
     const T ti = (T)inv, ta = (T)a2;
     const C nv = mk<C>(ti * (old.x - ta * nh.x), ti * (old.y - ta * nh.y));
@@ -307,16 +308,16 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   __builtin_amdgcn_sched_barrier(0);
   if (!xreal && act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) pp[(t + m * (L / E)) * rs] = v[m];
+    for (int m = 0; m < E; ++m) pp[m * rstep] = v[m];
   }
 
   // back along axis 0 (unnormalised): the new state in the axis-0 form, the
   // working buffer's input of the next passes in the k-space form
   fft_line<T, L, E, +1>(v, t, lds, TileIdx<TC, PADT>{ci}, twl);
-  C* out = xreal ? pp : work + loc.off;
+  C* out = xreal ? pp : work + loc.off + r0;
   if (act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) out[(t + m * (L / E)) * rs] = v[m];
+    for (int m = 0; m < E; ++m) out[m * rstep] = v[m];
   }
 }
 
