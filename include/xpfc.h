@@ -183,6 +183,13 @@ int xpfc_get_variants(const xpfc_ctx* ctx, int* v);
  * 50 steps and replayed (on by default; XPFC_GRAPHS=0 or on = 0 launches every
  * kernel eagerly).  Both give identical fields. */
 int xpfc_set_graphs(xpfc_ctx* ctx, int on);
+/* State form (chosen when a context is made; not an entry point): between
+ * steps the state is held as psi_k, or, for 3-D grids whose spectrum is at
+ * least 16 MiB, as psi_k transformed back along axis 0 (the axis-0 pass then
+ * writes one spectrum less per step).  The two agree to the rounding of one
+ * transform pair per step; XPFC_STATE_FORM=0 / 1 in the environment forces
+ * psi_k / the axis-0 form.  Checkpoints record the form and load only into a
+ * context of the same form. */
 /* 1 when all work queued on the context has finished, 0 while it runs
  * (never blocks), minus a status code on failure. */
 int xpfc_query(xpfc_ctx* ctx);
