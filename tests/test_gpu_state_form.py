@@ -161,3 +161,31 @@ def test_default_form_choice(monkeypatch):
         assert np.max(np.abs(back - psi0)) <= (1e-15 if prec == XPFC_FP64 else 1e-7)
         ref, _ = O.run(psi0, op, 2, sample=False)
         assert np.max(np.abs(f - ref)) <= (1e-12 if prec == XPFC_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("form", [1], indirect=True)
+@pytest.mark.parametrize("prec", PRECS)
+def test_axis0_form_budgeted_blocks_equal_alone(form, prec):
+    """The service's budgeted blocks (each instance its own number of steps in
+    one captured block) in the axis-0 form: every field equals its run alone."""
+    import ctypes as ct
+    from xpfc_cuda_amd import lib
+    _flush("axis-0 form budgeted blocks prec=%d" % prec)
+    ps = []
+    for i, sh in enumerate([(32, 32, 32), (24, 40, 48), (32, 32, 32)]):
+        _, p = make(sh, prec=prec, sigma=0.08 + 0.01 * i, psi0=0.05, t_sample=0)
+        ps.append(p)
+    fields = [0.05 + 0.02 * np.random.default_rng(60 + i).standard_normal(p.n) for i, p in enumerate(ps)]
+    budgets, lens = [[50, 7, 0], [13, 32, 32]], [50, 32]
+    with MixedContext(ps) as mc:
+        for s, x in enumerate(fields):
+            mc.set_field(s, x)
+        for b, n in zip(budgets, lens):
+            assert lib().xpfc_step_budget(mc.handle, n, (ct.c_int * len(ps))(*b)) == 0
+        mc.sync()
+        got = [mc.get_field(s) for s in range(len(ps))]
+    for s, p in enumerate(ps):
+        with Context(p) as c:
+            c.set_field(fields[s])
+            c.step(budgets[0][s] + budgets[1][s])
+            assert np.array_equal(got[s], c.get_field()), s
