@@ -140,13 +140,15 @@ __device__ __forceinline__ void strided_body(const cpx<T>* in, cpx<T>* out, cons
   const TileLoc loc = locate(segs, tile, ci, TC);
   const int rs = loc.rs;
   // row t + m * L/E of the column: the thread's offset plus a uniform step
-  const int r0 = t * rs, rstep = (L / E) * rs;  // (< 2^31 for every supported grid)
+  // (t * rs and (L/E) * rs stay below 2^31 on every supported grid; m times
+  // the step may not, at 2048 points along axis 0 and large planes)
+  const int r0 = t * rs, rstep = (L / E) * rs;
   const C* src = in + loc.off + r0;
   C* p = out + loc.off + r0;
   C v[E];
 #pragma unroll
   for (int m = 0; m < E; ++m) {
-    const C* q = src + m * rstep;
+    const C* q = src + (long long)m * rstep;
     v[m] = loc.act ? (NT ? load_nt(q) : *q) : mk<C>(0, 0);
   }
   stage_table<tile_threads<L, TC>()>(twl, tw, L);  // read after the first exchange's barrier
@@ -154,7 +156,7 @@ __device__ __forceinline__ void strided_body(const cpx<T>* in, cpx<T>* out, cons
   if (loc.act) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
-      C* q = p + m * rstep;
+      C* q = p + (long long)m * rstep;
       if constexpr (NT) store_nt(q, v[m]);
       else *q = v[m];
     }
@@ -213,9 +215,9 @@ __device__ __forceinline__ void xload(XTile<T, L, elems_per_thread<L>()>& x, con
   const C* pw = work + x.loc.off + r0;
   const C* pp = psi + x.loc.off + r0;
 #pragma unroll
-  for (int m = 0; m < E; ++m) x.v[m] = x.loc.act ? pw[m * rstep] : mk<C>(0, 0);
+  for (int m = 0; m < E; ++m) x.v[m] = x.loc.act ? pw[(long long)m * rstep] : mk<C>(0, 0);
 #pragma unroll
-  for (int m = 0; m < E; ++m) x.ps[m] = x.loc.act ? pp[m * rstep] : mk<C>(0, 0);
+  for (int m = 0; m < E; ++m) x.ps[m] = x.loc.act ? pp[(long long)m * rstep] : mk<C>(0, 0);
 }
 
 template <typename T, int L, int TC, bool SAMPLE, bool LUT>
@@ -308,7 +310,7 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   __builtin_amdgcn_sched_barrier(0);
   if (!xreal && act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) pp[m * rstep] = v[m];
+    for (int m = 0; m < E; ++m) pp[(long long)m * rstep] = v[m];
   }
 
   // back along axis 0 (unnormalised): the new state in the axis-0 form, the
@@ -317,7 +319,7 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   C* out = xreal ? pp : work + loc.off + r0;
   if (act) {
 #pragma unroll
-    for (int m = 0; m < E; ++m) out[m * rstep] = v[m];
+    for (int m = 0; m < E; ++m) out[(long long)m * rstep] = v[m];
   }
 }
 
