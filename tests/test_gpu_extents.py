@@ -165,3 +165,35 @@ def test_crystallising_radix3_radix5_vs_oracle(n, nsteps, prec):
     assert set(rfe) == set(gfe) and rfe
     for st, r in rfe.items():
         assert abs(gfe[st] - r) <= (1e-10 if fp64 else 1e-5) * abs(r), (st, gfe[st], r)
+
+
+def test_largest_planes_both_state_forms(monkeypatch):
+    """2048 x 2048 x 1280 FP32 (64 GB of spectra): 2048-point axis-0 lines over
+    planes of 1.3M elements, where a line's row offsets pass 2^31 elements.
+    From device noise, three steps in each state form: finite, mass exact,
+    the free energy falling step by step (a stable liquid), and the two forms
+    agreeing to FP32 rounding.  (No host copy of the 5.4G-point field.)"""
+    import math
+    n = (2048, 2048, 1280)
+    k1 = 2 * math.pi
+    out = {}
+    for form in ("0", "1"):
+        monkeypatch.setenv("XPFC_STATE_FORM", form)
+        monkeypatch.setenv("XPFC_AUTOTUNE", "0")
+        _flush("largest planes %s form %s" % (n, form))
+        _, p = make(n, sigma=0.1, psi0=0.05, peaks=[(k1, 1.0, 1.0), (k1 * 2.0 / math.sqrt(3.0), 1.0, 1.0)],
+                    prec=XPFC_FP32_MIXED, t_sample=0)
+        Fs = []
+        with Context(p) as ctx:
+            ctx.initialize_noise(0.01, 3)
+            Fs.append(ctx.fe_calc()[0])
+            for _ in range(3):
+                ctx.step(1)
+                Fs.append(ctx.fe_calc()[0])
+                assert abs(ctx.avg_calc() - 0.05) <= 1e-15
+        _flush("largest planes form %s F %s" % (form, Fs))
+        assert all(math.isfinite(F) for F in Fs)
+        assert all(b < a for a, b in zip(Fs, Fs[1:]))
+        out[form] = Fs
+    for a, b in zip(out["0"], out["1"]):
+        assert abs(a - b) <= 1e-6 * abs(a)
