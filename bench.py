@@ -357,8 +357,8 @@ def serving_run(seconds, rate, frac_3d=0.5, seed=0, budget=200e9, class_cells=1 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -472,7 +472,7 @@ def main():
     out["step_frac_of_copy"] = round(step_gbs / copy, 4)
     if not args.no_fp64:
         fsteps = max(10, args.steps // 4)
-        ms64, _, passes64, _, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 10))
+        ms64, _, passes64, _, _, _ = gpu_run(n, 0, fsteps, min(args.warmup, 50))
         v64 = 1000.0 / (ms64 / fsteps)
         out["fp64_steps_per_s"] = round(v64, 3)
         out["fp32_over_fp64"] = round(value / v64, 3)
