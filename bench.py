@@ -255,9 +255,10 @@ def config_runs():
 
 def batched_run(steps=2000, ninst=64, seed=0):
     """The 64-instance batch of 2-D grids of four sizes in one mixed-grid
-    context: every step is one launch per pass over all 64 instances, and
-    unsampled runs replay 50-step graphs of those launches (FE / RMSE every
-    100 steps).  Wall time of the whole job, and beside it the same job as one
+    context: every step is one launch per pass over the instances of each of
+    two stream groups (every grid's instances split between them, so their
+    launches overlap), and unsampled runs replay 50-step graphs of those
+    launches (FE / RMSE every 100 steps).  Wall time of the whole job, and beside it the same job as one
     context per grid size on its own stream (the classes overlap on the GPU)."""
     import xpfc_cuda_amd as X
     from xpfc_cuda_amd import serving as S
@@ -298,7 +299,8 @@ def batched_run(steps=2000, ninst=64, seed=0):
         "instances": ninst, "steps": steps, "wall_s": round(wall, 4),
         "instance_steps_per_s": round(ninst * steps / wall, 1),
         "cell_updates_per_s": cells * steps / wall,
-        "launches": "one per pass over all grids (mixed-grid context), 50-step graphs",
+        "launches": "one per pass over all grids in each of two stream groups (mixed-grid context; every "
+                    "grid's instances split between the groups), 50-step graphs per group",
         "sides": {str(side): sum(j.side == side for j in draws) for side in sorted({j.side for j in draws})},
         "last_fe_first_instance": fe, "nonfinite_instances": diverged,
     }

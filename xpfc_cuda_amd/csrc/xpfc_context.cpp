@@ -193,6 +193,9 @@ void free_all(xpfc_ctx* c) {
   for (XInst& in : c->inst)
     if (in.c2tab) (void)hipFree(in.c2tab);
   if (c->stream && !c->borrowed_stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
+  if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
+  if (c->ev_join) (void)hipEventDestroy(c->ev_join);
   if (c->stage) (void)hipHostFree(c->stage);
 }
 
@@ -1980,17 +1983,24 @@ int round8(int n) { return (n + 7) & ~7; }
 // Groups the classes of one pass into launches by (line family, LUT form),
 // numbering each launch's workgroups from 0 (wg0 of the strided passes on a
 // multiple of 8, so a workgroup's XCD is its index in the class mod 8).
-void rag_group(std::vector<std::pair<int, RClass>>& in, bool strided, std::vector<RClass>& all,
+struct RagEntry {
+  int grp;   // stream group
+  RClass r;  // the class (or half of its active list)
+};
+
+void rag_group(std::vector<RagEntry>& in, bool strided, std::vector<RClass>& all,
                std::vector<xpfc_ctx::RagLaunch>& out, int prec) {
+  for (int grp : {0, 1})
   for (int fam : {1, 3, 5}) {
     for (int lut : {1, 0}) {
       xpfc_ctx::RagLaunch l;
+      l.grp = grp;
       l.fam = fam;
       l.lut = lut != 0;
       l.first = (int)all.size();
       for (auto& e : in) {
-        RClass r = e.second;
-        if (line_family(r.L) != fam || (r.cubic != 0) != l.lut) continue;
+        RClass r = e.r;
+        if (e.grp != grp || line_family(r.L) != fam || (r.cubic != 0) != l.lut) continue;
         r.wg0 = strided ? round8(l.nwg) : l.nwg;
         l.nwg = r.wg0 + r.nact * r.wgs;
         l.lds = std::max(l.lds, strided ? rag_strided_lds(prec, r.L) : rag_zpass_lds(prec, r.L));
@@ -2013,10 +2023,28 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     return XPFC_OK;
   }
   xpfc_ctx::RagTables t;
-  std::vector<std::pair<int, RClass>> X, Y, Z;  // (class, entry) of each pass
+  std::vector<RagEntry> X, Y, Z;  // the entries of each pass
   const int nc = (int)c->cls.size();
   t.nbx.assign(nc, 0);
   t.nbz.assign(nc, 0);
+  // Each class's active list is split in halves between the two stream
+  // groups (a single instance goes to the group with less work so far).
+  const bool two = c->mixed_groups == 2 && c->stream2;
+  double work[2] = {0.0, 0.0};
+  // the entries of one pass for a class: one per group that takes a share
+  // (h[g] instances, the group-0 share first in the class's active list)
+  auto add = [&](std::vector<RagEntry>& list, const RClass& base, const int* h, bool strided_lgc) {
+    int first = 0;
+    for (int g = 0; g < 2; ++g) {
+      if (h[g] == 0) continue;
+      RClass e = base;
+      e.nact = h[g];
+      e.bt.slots = base.bt.slots + first;
+      if (strided_lgc) e.lgc = rag_lgc(h[g] * base.wgs);
+      list.push_back({g, e});
+      first += h[g];
+    }
+  };
   for (int k = 0; k < nc; ++k) {
     xpfc_ctx* q = c->cls[k];
     const int na = (int)q->active.size();
@@ -2027,6 +2055,16 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
       return fail(XPFC_ERR_ARG, "grid %d x %d x %d has no one-launch form", q->n0, q->n1, q->n2);
     if (na == 0) continue;
     const Batch bt = bt_active(q);
+    // the class's share per group
+    int h[2] = {na, 0};
+    const double per = (double)q->stride * q->csize;
+    if (two) {
+      const int lo = work[0] <= work[1] ? 0 : 1;
+      h[lo] = (na + 1) / 2;
+      h[1 - lo] = na - h[lo];
+    }
+    work[0] += per * h[0];
+    work[1] += per * h[1];
     RClass r{};
     r.L = q->n0;
     r.lgc = rag_lgc(na * t.nbx[k]);
@@ -2041,7 +2079,7 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     r.tw = q->tw0;
     r.partial = q->px;
     r.budget = budget ? q->d_budget : nullptr;
-    X.push_back({k, r});
+    add(X, r, h, true);
     if (q->n1 > 1) {
       RClass y{};
       y.L = q->n1;
@@ -2054,7 +2092,7 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
       y.b = q->g.xreal ? q->psi : q->work;  // the inverse pass reads the state in the axis-0 form
       y.tw = q->tw1;
       y.budget = r.budget;
-      Y.push_back({k, y});
+      add(Y, y, h, false);
     }
     RClass z{};
     z.L = q->H;
@@ -2071,9 +2109,10 @@ int mixed_tables(xpfc_ctx* c, bool budget, const xpfc_ctx::RagTables** out) {
     z.zl = ZLines{q->n1, q->xs, q->xs};
     z.nyoff = q->nyoff;
     z.budget = r.budget;
-    Z.push_back({k, z});
+    add(Z, z, h, false);
   }
-  if ((int)Z.size() > RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids active at once", RAG_MAX);
+  if ((int)Z.size() > 2 * RAG_MAX) return fail(XPFC_ERR_ARG, "more than %d grids active at once", RAG_MAX);
+  t.two = work[1] > 0.0;
   std::vector<RClass> all;
   rag_group(X, true, all, t.x, c->prec);
   rag_group(Y, true, all, t.y, c->prec);
@@ -2102,16 +2141,40 @@ bool mixed_pw(const xpfc_ctx* c) {
 // cubic and other boxes), axis 1 inverse (3-D classes), real space, axis 1
 // forward (3-D classes), then for sampled steps each listed class's finalize
 // into its row `rows[k]`.
+// The second group's work runs on stream2, forked from and joined back into
+// the context stream (also inside a capture, where the two become branches of
+// the graph).
+int mixed_fork(xpfc_ctx* c, const xpfc_ctx::RagTables& t) {
+  if (!t.two) return XPFC_OK;
+  HIPCHK(hipEventRecord(c->ev_fork, c->stream));
+  HIPCHK(hipStreamWaitEvent(c->stream2, c->ev_fork, 0));
+  return XPFC_OK;
+}
+int mixed_join(xpfc_ctx* c, const xpfc_ctx::RagTables& t) {
+  if (!t.two) return XPFC_OK;
+  HIPCHK(hipEventRecord(c->ev_join, c->stream2));
+  HIPCHK(hipStreamWaitEvent(c->stream, c->ev_join, 0));
+  return XPFC_OK;
+}
+
 int mixed_one_step(xpfc_ctx* c, const xpfc_ctx::RagTables& t, bool sample, int sib,
-                   const std::vector<int>* rows, bool pw) {
+                   const std::vector<int>* rows, bool pw, bool forkjoin = true, int only = -1) {
+  if (forkjoin)
+    if (int rc = mixed_fork(c, t)) return rc;
+  auto st = [&](const xpfc_ctx::RagLaunch& l) { return l.grp ? c->stream2 : c->stream; };
+  auto mine = [&](const xpfc_ctx::RagLaunch& l) { return only < 0 || l.grp == only; };
   for (const auto& l : t.x)
-    LCHK(launch_xstep_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, l.lut, sample, sib, c->stream));
+    if (mine(l))
+      LCHK(launch_xstep_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, l.lut, sample, sib, st(l)));
   for (const auto& l : t.y)
-    LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, +1, sib, c->stream));
+    if (mine(l)) LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, +1, sib, st(l)));
   for (const auto& l : t.z)
-    LCHK(launch_zpass_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, sample, pw, sib, c->stream));
+    if (mine(l))
+      LCHK(launch_zpass_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, sample, pw, sib, st(l)));
   for (const auto& l : t.y)
-    LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, -1, sib, c->stream));
+    if (mine(l)) LCHK(launch_strided_ragged(c->prec, l.fam, t.d + l.first, l.ncls, l.nwg, l.lds, -1, sib, st(l)));
+  if (forkjoin)
+    if (int rc = mixed_join(c, t)) return rc;
   if (sample && rows) {
     for (size_t k = 0; k < c->cls.size(); ++k) {
       if ((*rows)[k] < 0) continue;
@@ -2144,30 +2207,45 @@ int mixed_plain(xpfc_ctx* c, const xpfc_ctx::RagTables& t, long long run, bool p
       len = 1;
       while (2 * len <= run) len *= 2;
     }
-    std::vector<int> key = active_counts(c);
-    key.push_back(len);
-    key.push_back(pw ? 1 : 0);
-    hipGraphExec_t ex = nullptr;
-    auto it = c->rgraphs.find(key);
-    if (it != c->rgraphs.end()) {
-      ex = it->second;
-    } else {
-      if (c->rgraphs.size() >= 64) mixed_drop_graphs(c);
+    // one graph per stream group, captured on that group's stream and
+    // launched there between a fork and a join
+    hipGraphExec_t ex[2] = {nullptr, nullptr};
+    for (int g = 0; g < (t.two ? 2 : 1); ++g) {
+      std::vector<int> key = active_counts(c);
+      key.push_back(len);
+      key.push_back(pw ? 1 : 0);
+      key.push_back(t.two ? 10 + g : 0);
+      auto it = c->rgraphs.find(key);
+      if (it != c->rgraphs.end()) {
+        ex[g] = it->second;
+        continue;
+      }
+      if (c->rgraphs.size() >= 64) {
+        mixed_drop_graphs(c);
+        ex[0] = nullptr;
+        g = -1;  // rebuild both
+        continue;
+      }
+      hipStream_t sg = g ? c->stream2 : c->stream;
       hipGraph_t graph = nullptr;
-      HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      HIPCHK(hipStreamBeginCapture(sg, hipStreamCaptureModeThreadLocal));
       int rc = XPFC_OK;
-      for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = mixed_one_step(c, t, false, i, nullptr, pw);
-      const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
+      for (int i = 0; i < len && rc == XPFC_OK; ++i)
+        rc = mixed_one_step(c, t, false, i, nullptr, pw, false, t.two ? g : -1);
+      const hipError_t ec = hipStreamEndCapture(sg, &graph);
       if (rc != XPFC_OK || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
         return rc != XPFC_OK ? rc : fail(XPFC_ERR_DEVICE, "step capture failed: %s", hipGetErrorString(ec));
       }
-      const hipError_t ei = hipGraphInstantiate(&ex, graph, nullptr, nullptr, 0);
+      const hipError_t ei = hipGraphInstantiate(&ex[g], graph, nullptr, nullptr, 0);
       (void)hipGraphDestroy(graph);
       if (ei != hipSuccess) return fail(XPFC_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(ei));
-      c->rgraphs[key] = ex;
+      c->rgraphs[key] = ex[g];
     }
-    HIPCHK(hipGraphLaunch(ex, c->stream));
+    if (int rc = mixed_fork(c, t)) return rc;
+    HIPCHK(hipGraphLaunch(ex[0], c->stream));
+    if (t.two) HIPCHK(hipGraphLaunch(ex[1], c->stream2));
+    if (int rc = mixed_join(c, t)) return rc;
     for (xpfc_ctx* q : c->cls) advance(q, len);
     run -= len;
   }
@@ -2185,8 +2263,10 @@ int mixed_budget_block(xpfc_ctx* c, int len) {
     HIPCHK(hipMemcpyAsync(q->d_budget, q->hbud.data(), q->hbud.size() * sizeof(int), hipMemcpyHostToDevice,
                           c->stream));
   if (!c->use_graphs) {
+    if (int rc = mixed_fork(c, *t)) return rc;
     for (int i = 0; i < len; ++i)
-      if (int rc = mixed_one_step(c, *t, false, i, nullptr, pw)) return rc;
+      if (int rc = mixed_one_step(c, *t, false, i, nullptr, pw, false)) return rc;
+    if (int rc = mixed_join(c, *t)) return rc;
   } else {
     std::vector<int> key = active_counts(c);
     key.push_back(len);
@@ -2200,8 +2280,9 @@ int mixed_budget_block(xpfc_ctx* c, int len) {
       if (c->rgraphs.size() >= 64) mixed_drop_graphs(c);
       hipGraph_t graph = nullptr;
       HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      int rc = XPFC_OK;
-      for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = mixed_one_step(c, *t, false, i, nullptr, pw);
+      int rc = mixed_fork(c, *t);
+      for (int i = 0; i < len && rc == XPFC_OK; ++i) rc = mixed_one_step(c, *t, false, i, nullptr, pw, false);
+      if (rc == XPFC_OK) rc = mixed_join(c, *t);
       const hipError_t ec = hipStreamEndCapture(c->stream, &graph);
       if (rc != XPFC_OK || ec != hipSuccess) {
         if (graph) (void)hipGraphDestroy(graph);
@@ -2347,6 +2428,14 @@ extern "C" int xpfc_create_mixed(const xpfc_params* ps, int K, xpfc_ctx** out) {
   if (hipSetDevice(ps[0].device) != hipSuccess)
     return bail(fail(XPFC_ERR_DEVICE, "hipSetDevice(%d) failed", ps[0].device));
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+    return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
+  // the second stream group (default priority, like the first); XPFC_MIXED_STREAMS=1 keeps one
+  const char* ms = std::getenv("XPFC_MIXED_STREAMS");
+  c->mixed_groups = (ms && ms[0] == '1' && ms[1] == 0) ? 1 : 2;
+  if (c->mixed_groups == 2 &&
+      (hipStreamCreateWithFlags(&c->stream2, hipStreamNonBlocking) != hipSuccess ||
+       hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+       hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess))
     return bail(fail(XPFC_ERR_DEVICE, "stream creation failed"));
   c->where.assign(K, {0, 0});
   for (size_t k = 0; k < members.size(); ++k) {

@@ -116,6 +116,7 @@ struct xpfc_ctx {
   std::vector<std::vector<int>> inst_of;  // class k, slot i -> instance number
   bool borrowed_stream = false;      // a class context: the stream belongs to its owner
   struct RagLaunch {                 // one ragged launch: classes of one line family
+    int grp = 0;                     // stream group: 0 on `stream`, 1 on `stream2`
     int fam = 1;                     // 1, 3 or 5 (2^k, 3*2^k, 5*2^k points)
     bool lut = false;                // axis 0: every class has a square / cubic box
     int first = 0, ncls = 0;         // its classes: d[first .. first + ncls)
@@ -126,7 +127,15 @@ struct xpfc_ctx {
     xpfc::RClass* d = nullptr;       // the classes of every launch, back to back
     std::vector<RagLaunch> x, y, z;  // axis 0, axis 1 (3-D classes), real space
     std::vector<int> nbx, nbz;       // per class: X / Z partial sums per instance
+    bool two = false;                // launches on both streams
   };
+  // Several-grid contexts step their instances in two groups, each class's
+  // active list split in halves, on `stream` and `stream2`: the groups'
+  // launches overlap, filling each other's ramp-up and tail.  Forked from and
+  // joined back into `stream` around every block of steps.
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  int mixed_groups = 2;              // 1 or 2 (XPFC_MIXED_STREAMS)
   std::map<std::vector<int>, RagTables> rtab;                  // key: active counts (+ budget flag)
   std::map<std::vector<int>, hipGraphExec_t> rgraphs;          // key: length, counts, flags
 };
