@@ -317,9 +317,11 @@ int xpfc_get_inst_params(const xpfc_ctx* ctx, int s, xpfc_params* out);
  * lengths; longer extents are refused with XPFC_ERR_ARG) but one precision
  * and device.  Every step advances all active instances of every grid with
  * one launch per pass and line-length family present (the axis-1 passes cover
- * the 3-D grids), and runs of
- * unsampled steps are replayed from captured hipGraphs as in one-grid
- * contexts, so a block of steps of the whole mix is one graph.  Each
+ * the 3-D grids) in each of two stream groups (every grid's instances split
+ * between them, a second stream forked from and joined back into the
+ * context's; XPFC_MIXED_STREAMS=1 keeps one), and runs of unsampled steps are
+ * replayed from captured hipGraphs as in one-grid contexts, so a block of
+ * steps of the whole mix is one graph per group.  Each
  * instance's field is bitwise the one a one-instance context gives; sampled
  * FE / RMSE records sum their partials in another grouping (equal to
  * rounding), and xpfc_inst_fe_calc is bitwise the same.  The xpfc_inst_*
