@@ -259,3 +259,53 @@ def test_budgeted_blocks_one_grid(prec, graphs):
         for s, p in enumerate(ps):
             a, _, _ = _alone(p, psis[s], totals[s])
             assert np.array_equal(got[s], a), "%s instance %d differs" % (shape, s)
+
+
+
+@pytest.mark.parametrize("prec", [XPFC_FP64, XPFC_FP32_MIXED])
+@pytest.mark.parametrize("graphs", [True, False])
+def test_one_and_two_stream_groups_agree(prec, graphs, monkeypatch):
+    """A several-grid context steps its instances in two stream groups (each
+    grid's instances split between the context stream and a second stream,
+    forked and joined every block); XPFC_MIXED_STREAMS=1 keeps one.  Both give
+    every instance the same field bit for bit and the same samples, through
+    sampled steps and plain blocks, and through budgeted blocks."""
+    import ctypes as ct
+    from xpfc_cuda_amd import lib
+    _flush("one vs two stream groups prec=%d graphs=%d" % (prec, graphs))
+    budgets = [[30, 4, 0, 30, 17, 9, 30], [12, 12, 5, 0, 12, 1, 7]]
+    out = []
+    for groups in ("1", "2"):
+        monkeypatch.setenv("XPFC_MIXED_STREAMS", groups)
+        ps = _mix(prec)
+        psis = [_field(tuple(p.n), 110 + s) for s, p in enumerate(ps)]
+        with MixedContext(ps) as mc:
+            mc.set_graphs(graphs)
+            for s, x in enumerate(psis):
+                mc.set_field(s, x)
+            mc.step(61)
+            mc.sync()
+            sampled = ([mc.get_field(s) for s in range(len(ps))], [mc.samples(s) for s in range(len(ps))])
+        qs = []
+        for p in _mix(prec):
+            p.t_sample = 0
+            qs.append(p)
+        with MixedContext(qs) as mc:
+            mc.set_graphs(graphs)
+            for s, x in enumerate(psis):
+                mc.set_field(s, x)
+            for b, n in zip(budgets, (30, 12)):
+                assert lib().xpfc_step_budget(mc.handle, n, (ct.c_int * len(qs))(*b)) == 0
+            mc.step(9)
+            mc.sync()
+            steps = [mc.current_step(s) for s in range(len(qs))]
+            budgeted = [mc.get_field(s) for s in range(len(qs))]
+        out.append((sampled, budgeted, steps))
+    assert out[0][2] == out[1][2] == [a + b + 9 for a, b in zip(*budgets)]
+    for s in range(len(budgets[0])):
+        for k in range(2):
+            a = out[0][0][0][s] if k == 0 else out[0][1][s]
+            b = out[1][0][0][s] if k == 0 else out[1][1][s]
+            assert np.all(np.isfinite(a))
+            assert np.array_equal(a, b), "instance %d differs (%s)" % (s, "sampled" if k == 0 else "budgeted")
+        _close_samples(out[0][0][1][s], out[1][0][1][s], 1e-12)
