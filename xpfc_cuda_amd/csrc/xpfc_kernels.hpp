@@ -240,17 +240,32 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   const int myz = jb * jb + kz * kz;  // LUT: integer part of k^2 from axes 1, 2
   // the line's C2 values are gathered with the input streams (gathering them
   // after the forward transform was measured slower, and for 12-point threads
-  // it raised the register count)
+  // it raised the register count).  Boxes that are not square / cubic evaluate
+  // them here too, while the loads are in flight: inside the update loop the
+  // FP64 exp temporaries met the transformed lines and took the 512-point FP32
+  // pass from 117 to 136 VGPRs, one workgroup per CU instead of two (512^3 with
+  // one side scaled: axis-0 pass 0.66 ms evaluated in the loop, 0.40 ms here).
   // (A per-m table of the update's FP32 coefficients for every mode, which
   // would save the FP64 chain, made the pass 22 % slower at 512^3: the C2
   // table is read only below the cutoff, 9 % of the modes, the full table for
   // every mode.)
-  double c2t[LUT ? E : 1];
+  const double ky = g.dk1 * (double)jb;
+  const double kzv = g.dk2 * (double)kz;
+  const double kyz2 = ky * ky + kzv * kzv;
+  double c2t[E];
   if constexpr (LUT) {
 #pragma unroll
     for (int m = 0; m < E; ++m) {
       const int ia = signed_index(t + m * (L / E), g.n0);
       c2t[m] = c2_lookup(__mul24(ia, ia) + myz, ph);  // |ia| <= n0 / 2: 24-bit products
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ia = signed_index(t + m * (L / E), g.n0);
+      const double kx = g.dk0 * (double)ia;
+      const double k2 = kx * kx + kyz2;
+      c2t[m] = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
     }
   }
   C (&v)[E] = x.v;
@@ -265,9 +280,6 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
   const T s0 = (T)g.inv_n0;
   const bool kzero = xreal && loc.kind == 0 && loc.col == 0;
 
-  const double ky = g.dk1 * (double)jb;
-  const double kzv = g.dk2 * (double)kz;
-  const double kyz2 = ky * ky + kzv * kzv;
   const double wgt = act ? ((loc.kind == 0 && kz == 0) || loc.kind == 1 ? 1.0 : 2.0) : 0.0;
 
   // psi_k' = (psi_k - dt M k^2 N_k) / den  (euler_step.cu:50-59).  The
@@ -288,7 +300,7 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
     } else {
       const double kx = g.dk0 * (double)ia;
       k2 = kx * kx + kyz2;
-      c2 = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+      c2 = c2t[m];
     }
     double a2, inv;
     update_coefs(k2, c2, ph, a2, inv);
