@@ -173,7 +173,10 @@ int xpfc_copy_bandwidth(int device, size_t bytes, int reps, double* gbs);
  * real-space pass (plus 1: no occupancy hint).  Variants regroup lines into
  * workgroups, reorder workgroups or change register allocation, and never
  * change the computed field.  xpfc_autotune times every variant on scratch
- * buffers, keeps the fastest (cached per precision and grid for the process)
+ * buffers (two spectra per tuned instance when free device memory holds them,
+ * else in place in the context's diagnostics buffer, so tuning allocates no
+ * spectra the device cannot hold), keeps the fastest (cached per precision
+ * and grid for the process)
  * and returns the choice in v (may be NULL); xpfc_step runs it on first use
  * unless XPFC_AUTOTUNE=0 or xpfc_set_variants was called (0 = keep). */
 int xpfc_autotune(xpfc_ctx* ctx, int* v);

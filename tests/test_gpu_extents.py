@@ -197,3 +197,63 @@ def test_largest_planes_both_state_forms(monkeypatch):
         out[form] = Fs
     for a, b in zip(out["0"], out["1"]):
         assert abs(a - b) <= 1e-6 * abs(a)
+
+
+def _hip_mem_info():
+    """(free, total) device bytes from the HIP runtime the library runs on."""
+    import ctypes as ct
+    path = None
+    with open("/proc/self/maps") as f:
+        for line in f:
+            if "libamdhip64.so" in line:
+                path = line.split()[-1]
+                break
+    assert path, "HIP runtime not loaded"
+    hip = ct.CDLL(path)
+    free, total = ct.c_size_t(), ct.c_size_t()
+    assert hip.hipMemGetInfo(ct.byref(free), ct.byref(total)) == 0
+    return free.value, total.value
+
+
+def test_autotune_in_place_when_device_memory_is_short(monkeypatch):
+    """The autotuner times the pass variants in two scratch spectra; when free
+    device memory cannot hold them beside the context (the largest grids,
+    whose three buffers fill most of the GPU) it times them in place in the
+    context's diagnostics buffer.  Here the memory is filled from outside
+    before the first step: tuning succeeds, later calls see no stale
+    allocation error, and the field equals an untuned run's bit for bit (the
+    variants only regroup lines)."""
+    from xpfc_cuda_amd.api import DeviceBuffer
+    shape = (160, 96, 128)   # a grid no other test tunes (tuning results are cached per grid)
+    _flush("autotune in place %s" % (shape,))
+    _, p = make(shape, prec=XPFC_FP32_MIXED, sigma=0.09, psi0=0.05, t_sample=10)
+    psi0 = 0.05 + 0.02 * np.random.default_rng(5).standard_normal(shape)
+    monkeypatch.delenv("XPFC_AUTOTUNE", raising=False)
+    with Context(p) as ctx:
+        ctx.set_field(psi0)
+        free, total = _hip_mem_info()
+        keep = total // 32    # less than the tuner's margin of total / 16
+        filler = DeviceBuffer(ctx, max(free - keep, 1))
+        try:
+            assert _hip_mem_info()[0] < total // 16
+            ctx.step(25)
+            ctx.sync()
+            tuned = ctx.variants()
+        finally:
+            filler.free()
+        got = ctx.get_field()
+        fe = [s.F_ref for s in ctx.samples() if s.has_fe]
+    monkeypatch.setenv("XPFC_AUTOTUNE", "0")
+    with Context(p) as ctx:
+        ctx.set_variants(*tuned)
+        ctx.set_field(psi0)
+        ctx.step(25)
+        ref = ctx.get_field()
+        fe_ref = [s.F_ref for s in ctx.samples() if s.has_fe]
+    with Context(p) as ctx:   # the default variants, untuned
+        ctx.set_field(psi0)
+        ctx.step(25)
+        untuned = ctx.get_field()
+    assert np.all(np.isfinite(got))
+    assert np.array_equal(got, ref) and np.array_equal(got, untuned)
+    assert fe == fe_ref and len(fe) >= 2

@@ -1307,11 +1307,21 @@ int autotune(xpfc_ctx* c) {
     }
   }
   const size_t sbytes = (size_t)ny * c->stride * c->csize;
+  // two scratch spectra per tuned instance, when free memory holds them with
+  // room to spare; otherwise (the largest grids, whose three buffers fill most
+  // of the device) the variants are timed in place in the context's
+  // diagnostics buffer, used as both operands: its contents are rebuilt by
+  // every call that reads them, and timing does not depend on the values
   void *a = nullptr, *b = nullptr;
-  if (hipMalloc(&a, sbytes) != hipSuccess || hipMalloc(&b, sbytes) != hipSuccess) {
+  size_t mfree = 0, mtotal = 0;
+  const bool room = hipMemGetInfo(&mfree, &mtotal) == hipSuccess && 2 * sbytes + (mtotal >> 4) <= mfree;
+  if (room && (hipMalloc(&a, sbytes) != hipSuccess || hipMalloc(&b, sbytes) != hipSuccess)) {
     if (a) (void)hipFree(a);
-    return fail(XPFC_ERR_NOMEM, "autotune scratch allocation failed");
+    a = b = nullptr;
+    (void)hipGetLastError();  // a refused allocation is not a launch error
   }
+  const bool own = a && b;
+  if (!own) a = b = c->aux;
   int rc = XPFC_OK;
   if (hipMemsetAsync(a, 0, sbytes, c->stream) != hipSuccess ||
       hipMemsetAsync(b, 0, sbytes, c->stream) != hipSuccess)
@@ -1351,8 +1361,10 @@ int autotune(xpfc_ctx* c) {
       }
   }
   (void)hipStreamSynchronize(c->stream);
-  (void)hipFree(a);
-  (void)hipFree(b);
+  if (own) {
+    (void)hipFree(a);
+    (void)hipFree(b);
+  }
   if (rc) return rc;
   c->tc_x = best[0];
   c->tc_y = best[1];
