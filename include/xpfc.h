@@ -257,10 +257,11 @@ int xpfc_read_vtk(const char* path, int* dims, double* spacing, double* host_rea
  * working directory like the reference. */
 int xpfc_inst_initialize_vtk(xpfc_ctx* ctx, int s, const char* path);
 /* Binary checkpoint of one instance: parameters, step, FP64 mean and the
- * device buffers (the state, psi_k transformed back along axis 0, and the
- * nonlinear-term work field), so a run
+ * device buffers (the state in the context's form, psi_k or psi_k
+ * transformed back along axis 0, and the nonlinear-term work field), so a run
  * resumed from it continues bit for bit.  Loading requires the same grid and
- * precision; the sample log is cleared. */
+ * precision; a context of the other state form converts the state with one
+ * axis-0 transform; the sample log is cleared. */
 int xpfc_inst_save_checkpoint(xpfc_ctx* ctx, int s, const char* path);
 int xpfc_inst_load_checkpoint(xpfc_ctx* ctx, int s, const char* path);
 int xpfc_save_checkpoint(xpfc_ctx* ctx, const char* path);

@@ -1913,9 +1913,9 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
     std::fclose(f);
     return fail(XPFC_ERR_ARG, "%s: grid, box or precision differ from the context's", path);
   }
-  if (h.state_form != c->g.xreal) {
+  if (h.state_form != 0 && h.state_form != 1) {
     std::fclose(f);
-    return fail(XPFC_ERR_ARG, "%s: the state is held in the other form (XPFC_STATE_FORM differs)", path);
+    return fail(XPFC_ERR_PARSE, "%s: unknown state form %d", path, (int)h.state_form);
   }
   ok = std::fread(buf.data(), 1, buf.size(), f) == buf.size();
   std::fclose(f);
@@ -1928,6 +1928,21 @@ extern "C" int xpfc_inst_load_checkpoint(xpfc_ctx* c, int s, const char* path) {
   HIPCHK(hipMemcpyAsync(inst_ptr(c, c->psi, s), buf.data(), bytes, hipMemcpyHostToDevice, c->stream));
   HIPCHK(hipMemcpyAsync(inst_ptr(c, c->work, s), buf.data() + bytes, bytes, hipMemcpyHostToDevice,
                         c->stream));
+  if (h.state_form != c->g.xreal) {
+    // written in the other state form (the working buffer has one form in
+    // both): psi_k -> state = unnormalised inverse FFT along axis 0, or
+    // state -> psi_k = FFT_0(state) / n0 with the k = 0 slot exactly 0, as the
+    // axis-0 pass forms it
+    const Batch bt = bt_one(c, s);
+    const int dir = c->g.xreal ? +1 : -1;
+    LCHK(launch_strided(c->prec, c->psi, x_segs(c, c->tc_x), c->n0, c->tc_x, dir, c->tw0, bt, 1, false,
+                        c->stream));
+    if (!c->g.xreal) {
+      void* ps = inst_ptr(c, c->psi, s);
+      LCHK(launch_scale(c->prec, ps, 2LL * c->stride, c->g.inv_n0, c->stream));
+      HIPCHK(hipMemsetAsync(ps, 0, c->csize, c->stream));
+    }
+  }
   HIPCHK(hipStreamSynchronize(c->stream));
   in.step = h.step;
   in.samples.clear();
