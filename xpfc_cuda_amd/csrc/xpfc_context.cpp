@@ -190,8 +190,10 @@ void free_all(xpfc_ctx* c) {
                   c->pz, c->sums, c->d_ia, c->d_ids, c->d_active, c->d_flags, c->d_budget};
   for (void* q : ptrs)
     if (q) (void)hipFree(q);
-  for (XInst& in : c->inst)
+  for (XInst& in : c->inst) {
     if (in.c2tab) (void)hipFree(in.c2tab);
+    if (in.c2rows) (void)hipFree(in.c2rows);
+  }
   if (c->stream && !c->borrowed_stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   if (c->ev_fork) (void)hipEventDestroy(c->ev_fork);
@@ -227,6 +229,27 @@ long long grid_mmax(int n0, int n1, int n2) {
 // or the largest m on the grid).
 int c2_table_entries(double kcut2, double dksq, int n0, int n1, int n2) {
   return (int)std::min(std::ceil(kcut2 / dksq) + 1.0, (double)grid_mmax(n0, n1, n2) + 1.0);
+}
+
+// Rows of the C2 table of a 2-D grid whose box is not square: for each kz
+// with k_z^2 below the cutoff, the modes |i| = 0 .. len-1, len one index past
+// where the cutoff falls on the host, so that the device's own comparison
+// (in the table builder, with the axis-0 pass's arithmetic) decides the edge
+// and every mode outside the rows is beyond the cutoff.
+std::vector<C2Row> c2_rows_2d(int n0, int H, double dk0, double dk2, double kcut2, long long* total) {
+  std::vector<C2Row> rows;
+  long long off = 0;
+  const double kc = kcut2 * (1.0 + 1e-9);
+  for (int l = 0; l <= H; ++l) {
+    const double kz = dk2 * (double)l;
+    const double rem = kc - kz * kz;
+    if (rem <= 0.0) break;
+    const long long len = std::min<long long>(n0 / 2 + 1, (long long)std::floor(std::sqrt(rem) / dk0) + 2);
+    rows.push_back(C2Row{(int)off, (int)len});
+    off += len;
+  }
+  *total = off;
+  return rows;
 }
 
 // The state form (KGeom::xreal) of a grid with n1 points along axis 1 (1 in
@@ -279,6 +302,34 @@ int build_inst(xpfc_ctx* c, const xpfc_params& p, XInst* in) {
     ph.mcut = mcut;
     ph.c2tab = in->c2tab;
     LCHK(launch_c2table(in->c2tab, mcut, c->g.dksq, ph, c->stream));
+  } else if (c->n1 == 1) {
+    // 2-D box that is not square: the rows over kz (its columns)
+    long long total = 0;
+    const std::vector<C2Row> rows = c2_rows_2d(c->n0, c->H, c->g.dk0, c->g.dk2, ph.kcut2, &total);
+    if (!rows.empty()) {
+      HIPCHK(hipStreamSynchronize(c->stream));  // queued steps may still read the old rows
+      if (total > in->c2cap) {
+        if (in->c2tab) (void)hipFree(in->c2tab);
+        in->c2tab = nullptr;
+        in->c2cap = 0;
+        if (hipMalloc((void**)&in->c2tab, (size_t)total * sizeof(double)) != hipSuccess)
+          return fail(XPFC_ERR_NOMEM, "C2 table allocation failed");
+        in->c2cap = (int)total;
+      }
+      if ((int)rows.size() > in->c2rcap) {
+        if (in->c2rows) (void)hipFree(in->c2rows);
+        in->c2rows = nullptr;
+        in->c2rcap = 0;
+        if (hipMalloc((void**)&in->c2rows, rows.size() * sizeof(C2Row)) != hipSuccess)
+          return fail(XPFC_ERR_NOMEM, "C2 row allocation failed");
+        in->c2rcap = (int)rows.size();
+      }
+      HIPCHK(hipMemcpy(in->c2rows, rows.data(), rows.size() * sizeof(C2Row), hipMemcpyHostToDevice));
+      ph.c2tab = in->c2tab;
+      ph.c2rows = in->c2rows;
+      ph.c2nl = (int)rows.size();
+      LCHK(launch_c2rows(in->c2tab, in->c2rows, ph.c2nl, c->g.dk0, c->g.dk2, ph, c->stream));
+    }
   }
   in->ia.za.a = p.a;
   in->ia.za.b = p.b;
@@ -527,9 +578,14 @@ extern "C" size_t xpfc_instance_bytes(const xpfc_params* p) {
   size_t c2bytes = 0;
   const double L0 = p->L[0], L1 = p->dim == 3 ? p->L[1] : L0, L2 = p->dim == 3 ? p->L[2] : p->L[1];
   const double dk0 = 2.0 * M_PI / L0, dk1 = 2.0 * M_PI / L1, dk2 = 2.0 * M_PI / L2;
-  if (dk0 == dk2 && dk1 == dk0 && p->npeaks >= 1 && p->npeaks <= XPFC_MAX_PEAKS)
+  if (dk0 == dk2 && dk1 == dk0 && p->npeaks >= 1 && p->npeaks <= XPFC_MAX_PEAKS) {
     c2bytes = (size_t)c2_table_entries(peak_kcut2(*p, peak_ecut(prec)), dk0 * dk0, n0, n1, n2) *
               sizeof(double);
+  } else if (n1 == 1 && p->npeaks >= 1 && p->npeaks <= XPFC_MAX_PEAKS) {  // the 2-D rows
+    long long total = 0;
+    const size_t nrows = c2_rows_2d(n0, H, dk0, dk2, peak_kcut2(*p, peak_ecut(prec)), &total).size();
+    c2bytes = (size_t)total * sizeof(double) + nrows * sizeof(C2Row);
+  }
   return 3 * (size_t)(nyoff + nny) * csize + ((size_t)npx * XPART + (size_t)npz * ZPART) * sizeof(double) +
          (size_t)(SUM_ROWS + 1) * NSUMS * sizeof(double) + 2 * sizeof(InstArgs) + 3 * sizeof(int) + c2bytes;
 }

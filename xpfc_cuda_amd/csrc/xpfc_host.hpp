@@ -54,8 +54,10 @@ struct XInst {
   bool have_rmse = false;
   bool nonfinite = false;           // a sampled sum of this instance was NaN / Inf
   bool nonfinite_reported = false;  // ... and xpfc_sync has reported it
-  double* c2tab = nullptr;  // device C2 table (square / cubic grids)
+  double* c2tab = nullptr;  // device C2 table (square / cubic grids, non-square 2-D boxes)
   int c2cap = 0;            // its capacity in entries
+  xpfc::C2Row* c2rows = nullptr;  // device row index of the non-square 2-D table
+  int c2rcap = 0;           // its capacity in rows
 };
 
 // One sampled step awaiting read-back: its row in the sums buffer and the

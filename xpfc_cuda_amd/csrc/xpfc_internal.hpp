@@ -44,6 +44,12 @@ struct KGeom {
 };
 
 // Physics constants of the semi-implicit update (all FP64).
+// One row of the C2 table of a 2-D box that is not square: the modes
+// |i| = 0 .. len-1 of column kz start at entry off.
+struct C2Row {
+  int off, len;
+};
+
 struct Phys {
   double dtM;        // dt * Mo
   double one_m_eps;  // 1 - eps
@@ -58,6 +64,10 @@ struct Phys {
   // cubic geometry: c2tab[m] = C2(sqrt(dksq * m)) for m < mcut, 0 beyond
   int mcut;
   const double* c2tab;
+  // 2-D box that is not square: c2tab[c2rows[kz].off + |i|] for kz < c2nl and
+  // |i| < c2rows[kz].len, 0 beyond (null: evaluated per mode)
+  const C2Row* c2rows;
+  int c2nl;
 };
 
 // Real-space pass constants.
@@ -237,6 +247,7 @@ int launch_kspace_sums(int prec, const void* psi, long long nmain, long long nny
                        const Batch& bt, double* partial, int* nblocks, hipStream_t s);
 // tab[m] = C2(sqrt(dksq * m)), m < mcut, for the physics `ph`
 int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s);
+int launch_c2rows(double* tab, const C2Row* rows, int nl, double dk0, double dk2, const Phys& ph, hipStream_t s);
 
 // operator-level kernels on natural layouts (real [N], spectrum [n0][n1][n2/2+1])
 int launch_power(int prec, const void* src, void* dst, long long n, int p, bool f32, hipStream_t s);

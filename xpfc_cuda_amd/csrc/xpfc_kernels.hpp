@@ -259,6 +259,15 @@ __device__ __forceinline__ void xtile(XTile<T, L, elems_per_thread<L>()>& x, int
       const int ia = signed_index(t + m * (L / E), g.n0);
       c2t[m] = c2_lookup(__mul24(ia, ia) + myz, ph);  // |ia| <= n0 / 2: 24-bit products
     }
+  } else if (ph.c2rows) {  // 2-D box that is not square: the instance's rows
+    C2Row r{0, 0};
+    if (kz < ph.c2nl) r = ph.c2rows[kz];
+#pragma unroll
+    for (int m = 0; m < E; ++m) {
+      const int ia = signed_index(t + m * (L / E), g.n0);
+      const int ai = ia < 0 ? -ia : ia;
+      c2t[m] = ai < r.len ? ph.c2tab[r.off + ai] : 0.0;
+    }
   } else {
 #pragma unroll
     for (int m = 0; m < E; ++m) {

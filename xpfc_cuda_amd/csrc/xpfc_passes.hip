@@ -216,6 +216,29 @@ __global__ __launch_bounds__(256) void k_c2table(double* __restrict__ tab, int m
   if (m < mcut) tab[m] = c2hat(sqrt(dksq * (double)m), ph);
 }
 
+// C2 rows of a non-square 2-D box, one workgroup per kz: k^2 formed as the
+// axis-0 pass forms it (k_y = 0), so each entry is the value that pass would
+// evaluate for the mode
+__global__ __launch_bounds__(256) void k_c2rows(double* __restrict__ tab, const C2Row* __restrict__ rows,
+                                                double dk0, double dk2, Phys ph) {
+  const int kz = blockIdx.x;
+  const C2Row r = rows[kz];
+  const double ky = 0.0;
+  const double kzv = dk2 * (double)kz;
+  const double kyz2 = ky * ky + kzv * kzv;
+  for (int i = threadIdx.x; i < r.len; i += 256) {
+    const double kx = dk0 * (double)i;
+    const double k2 = kx * kx + kyz2;
+    tab[r.off + i] = (k2 < ph.kcut2) ? c2hat(sqrt(k2), ph) : 0.0;
+  }
+}
+
+int launch_c2rows(double* tab, const C2Row* rows, int nl, double dk0, double dk2, const Phys& ph, hipStream_t s) {
+  if (nl <= 0) return 0;
+  hipLaunchKernelGGL(k_c2rows, dim3(nl), dim3(256), 0, s, tab, rows, dk0, dk2, ph);
+  return launch_status();
+}
+
 int launch_c2table(double* tab, int mcut, double dksq, const Phys& ph, hipStream_t s) {
   if (mcut <= 0) return 0;
   hipLaunchKernelGGL(k_c2table, dim3((mcut + 255) / 256), dim3(256), 0, s, tab, mcut, dksq, ph);
